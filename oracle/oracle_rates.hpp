@@ -1,0 +1,579 @@
+// TEST INFRASTRUCTURE — part of the CPU oracle.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load it, and only as the checker.  The product path never
+// links or calls anything under oracle/.
+//
+// FP64 restatement of the SPH-EXA-nnet reaction-rate parameterizations.  Every arithmetic
+// expression keeps the reference's evaluation tree (operand order and grouping), so that with
+// the same compiler flags (-O2 -ffp-contract=off, x86-64 baseline, glibc libm) the results are
+// bit-identical to the reference build in oracle/_ref.
+//
+//   net14 rates:  parameterization/net14/net14.cpp:40-660
+//   net86 rates:  parameterization/net86/net86.cpp:41-686
+//   net87 rates:  parameterization/net87/net87.hpp:117-175, net87/net87.cpp:102-141
+//   Coulomb fits: net14/net14_tables.hpp:46-67 (ggt1/glt1), T-bins :71-101
+#pragma once
+#include <algorithm>
+#include <cmath>
+
+#include "../sph-exa-nnet_amd/csrc/nnet_tables.h"
+
+namespace oracle
+{
+namespace phys
+{
+// parameterization/net14/net14_constants.hpp:45-50 (net86_constants.hpp:51-56 are identical)
+const double pi       = 3.14159265358979323846264338327950288;
+const double Kb       = 1.380658e-16;
+const double Na       = 6.022137e23;
+const double e2       = 2.306022645e-19;
+const double MevToErg = 9.648529392e17;
+} // namespace phys
+
+// net14_tables.hpp:71-101 (same bins in net86_tables.hpp:46-76)
+inline int temperature_bin(double T)
+{
+    static const double edges[23] = {1.5e8, 2e8, 3e8,   4e8, 5e8,   6e8, 7e8,   8e8, 9e8, 1e9, 1.5e9, 2e9,
+                                     2.5e9, 3e9, 3.5e9, 4e9, 4.5e9, 5e9, 6e9,   7e9, 8e9, 9e9, 1e10};
+    int k = 0;
+    while (k < 23 && !(T < edges[k]))
+        ++k;
+    return k;
+}
+
+// net14_tables.hpp:47-57
+inline double coulomb_strong(double x)
+{
+    const double a = -.898004, b = .96786, c = .220703, d = -.86097;
+    const double r = std::sqrt(std::sqrt(x));
+    return a * x + b * r + c / r + d;
+}
+// net14_tables.hpp:59-67
+inline double coulomb_weak(double x)
+{
+    const double a = -.5 * std::sqrt(3.), b = .29561, c = 1.9885;
+    return a * x * std::sqrt(x) + b * std::pow(x, c);
+}
+
+// Temperature-dependent corrected binding energies, net14.cpp:49-75 / net86.cpp:50-73.
+inline void corrected_binding_energies(int ns, const double* BE0, const double* Z, double T, double rho, double* BE)
+{
+    const double kbt        = phys::Kb * T;
+    const double nakbt      = phys::Na * kbt;
+    const double correction = -1.5 * nakbt;
+    for (int s = 0; s < ns; ++s)
+        BE[s] = BE0[s] + correction;
+    const double ne  = rho * phys::Na / 2.;
+    const double ae  = std::pow((3. / 4.) / (phys::pi * ne), 1. / 3.);
+    const double gam = phys::e2 / (kbt * ae);
+    for (int s = 0; s < ns; ++s)
+    {
+        const double g = gam * std::pow(Z[s], 5. / 3.);
+        const double f = g > 1 ? coulomb_strong(g) : coulomb_weak(g);
+        BE[s] -= nakbt * f;
+    }
+}
+
+// Screening chemical potentials mu_i/kT, net14.cpp:577-600 / net86.cpp:577-600.
+inline void screening_mu(int ns, const double* Z, double T, double rho, double* mukbt)
+{
+    const double ne  = rho * phys::Na / 2.;
+    const double kbt = T * phys::Kb;
+    const double ae  = std::pow(3. / (4. * phys::pi * ne), 1. / 3.);
+    const double gam = phys::e2 / (kbt * ae);
+    const double a1 = -.898004, b1 = .96786, c1 = .220703, d1 = -.86097, e1 = 2.520058332;
+    for (int s = 0; s < ns; ++s)
+    {
+        const double gp  = gam * std::pow(Z[s], 5. / 3.);
+        const double sq  = std::sqrt(gp);
+        const double sq2 = std::sqrt(sq);
+        if (gp <= 1)
+            mukbt[s] = -(1. / std::sqrt(3.)) * gp * sq + std::pow(gp, 1.9885) * .29561 / 1.9885;
+        else
+            mukbt[s] = a1 * gp + 4. * (b1 * sq2 - c1 / sq2) + d1 * std::log(gp) - e1;
+    }
+}
+
+// The hand-fitted rates shared by net14 and net86: 3-alpha (and reverse), 12C+12C, 12C+16O,
+// 16O+16O, 12C(a,g) (and reverse), 16O(a,g) (and reverse), with their temperature
+// derivatives.  Rates: net14.cpp:194-335 / net86.cpp:147-288 (t9 clamped at 10).
+// Derivatives: net14.cpp:339-567 / net86.cpp:292-519 (t9 not clamped).
+struct SpecialRates
+{
+    double r3a, rg3a, r24, r1216, r32, rcag, roga, roag, rnega;
+    double dr3a, drg3a, dr24, dr1216, dr32, drcag, droga, droag, drnega;
+};
+
+inline SpecialRates special_rates(double T)
+{
+    SpecialRates o{};
+    {
+        const double t9r  = T * 1.0e-09;
+        const double t9   = std::min(10., t9r);
+        const double t92  = t9 * t9;
+        const double t93  = t92 * t9;
+        const double t95  = t92 * t93;
+        const double t912 = std::sqrt(t9);
+        const double t913 = std::pow(t9, 1. / 3.);
+        const double t923 = t913 * t913;
+        const double t932 = t9 * t912;
+        const double t943 = t9 * t913;
+        const double t953 = t9 * t923;
+
+        const double t9i   = 1. / t9;
+        const double t9i13 = 1. / t913;
+        const double t9i23 = 1. / t923;
+        const double t9i32 = 1. / t932;
+        const double t9rm1 = 1. / t9r;
+        const double t9r32 = t9 * std::sqrt(t9r);
+
+        // triple alpha and its reverse
+        const double r2abe = (7.40e+05 * t9i32) * std::exp(-1.0663 * t9i) +
+                             4.164e+09 * t9i23 * std::exp(-13.49 * t9i13 - t92 / 0.009604) *
+                                 (1. + 0.031 * t913 + 8.009 * t923 + 1.732 * t9 + 49.883 * t943 + 27.426 * t953);
+        const double rbeac = (130. * t9i32) * std::exp(-3.3364 * t9i) +
+                             2.510e+07 * t9i23 * std::exp(-23.57 * t9i13 - t92 / 0.055225) *
+                                 (1. + 0.018 * t913 + 5.249 * t923 + 0.650 * t9 + 19.176 * t943 + 6.034 * t953);
+        if (T > 8e7)
+            o.r3a = 2.90e-16 * (r2abe * rbeac) + 0.1 * 1.35e-07 * t9i32 * std::exp(-24.811 * t9i);
+        else
+            o.r3a = 2.90e-16 * (r2abe * rbeac) *
+                        (0.01 + 0.2 * (1. + 4. * std::exp(-std::pow(0.025 * t9i, 3.263))) /
+                                    (1. + 4. * std::exp(-std::pow(t9 / 0.025, 9.227)))) +
+                    0.1 * 1.35e-07 * t9i32 * std::exp(-24.811 * t9i);
+        const double rev = 2.e20 * std::exp(-84.419412e0 * t9i);
+        o.rg3a           = o.r3a * rev * t93;
+
+        // 12C + 12C
+        {
+            const double t9a   = t9 / (1. + 0.0396 * t9);
+            const double t9a13 = std::pow(t9a, 1. / 3.);
+            const double t9a56 = std::pow(t9a, 5. / 6.);
+            o.r24              = 4.27e+26 * t9a56 * t9i32 * std::exp(-84.165 / t9a13 - 2.12e-03 * t93);
+        }
+        // 12C + 16O
+        o.r1216 = 0;
+        if (T >= 5e8)
+        {
+            const double t9ap    = t9 / (1. + 0.055 * t9);
+            const double t9a2p   = t9ap * t9ap;
+            const double t9a13p  = std::pow(t9ap, 1. / 3.);
+            const double t9a23p  = t9a13p * t9a13p;
+            const double t9a56ap = std::pow(t9ap, 5. / 6.);
+            o.r1216              = 1.72e+31 * t9a56ap * t9i32 * std::exp(-106.594 / t9a13p) /
+                      (std::exp(-0.18 * t9a2p) + 1.06e-03 * std::exp(2.562 * t9a23p));
+        }
+        // 16O + 16O
+        o.r32 = 7.10e+36 * t9i23 * std::exp(-135.93 * t9i13 - 0.629 * t923 - 0.445 * t943 + 0.0103 * t92);
+        // 12C(a,g)16O and reverse
+        o.rcag = 1.04e+08 / (t92 * std::pow(1. + 0.0489 * t9i23, 2.)) * std::exp(-32.120 * t9i13 - t92 / 12.222016) +
+                 1.76e+08 / (t92 * std::pow(1. + 0.2654 * t9i23, 2.)) * std::exp(-32.120 * t9i13) +
+                 1.25e+03 * t9i32 * std::exp(-27.499 * t9i) + 1.43e-02 * t95 * std::exp(-15.541 * t9i);
+        o.roga = o.rcag * 5.13e+10 * t9r32 * std::exp(-83.108047 * t9rm1);
+        // 16O(a,g)20Ne and reverse
+        o.roag  = (9.37e+09 * t9i23 * std::exp(-39.757 * t9i13 - t92 / 2.515396) +
+                  62.1 * t9i32 * std::exp(-10.297 * t9i) + 538. * t9i32 * std::exp(-12.226 * t9i) +
+                  13. * t92 * std::exp(-20.093 * t9i));
+        o.rnega = o.roag * 5.65e+10 * t9r32 * std::exp(-54.93807 * t9rm1);
+    }
+    {
+        const double t9    = T * 1.0e-09;
+        const double t92   = t9 * t9;
+        const double t93   = t92 * t9;
+        const double t94   = t93 * t9;
+        const double t95   = t94 * t9;
+        const double t912  = std::sqrt(t9);
+        const double t913  = std::pow(t9, 1. / 3.);
+        const double t923  = t913 * t913;
+        const double t932  = t9 * t912;
+        const double t943  = t9 * t913;
+        const double t952  = t9 * t932;
+        const double t953  = t9 * t923;
+        const double t9i   = 1. / t9;
+        const double t9i2  = t9i * t9i;
+        const double t9i13 = 1. / t913;
+        const double t9i23 = 1. / t923;
+        const double t9i32 = 1. / t932;
+        const double t9i43 = 1. / t943;
+        const double t9i52 = 1. / t952;
+        const double t9i53 = 1. / t953;
+
+        // d(triple alpha)/dT
+        {
+            const double vA  = -24.811 * t9i;
+            const double vB  = -1.0663 * t9i;
+            const double vC  = -13.49 * t9i13 - t92 * 104.123282;
+            const double vD  = 1. + .031 * t913 + 8.009 * t923 + 1.732 * t9 + 49.883 * t943 + 27.426 * t953;
+            const double vE  = -3.3364 * t9i;
+            const double vF  = -23.57 * t9i13 - 18.10774106 * t92;
+            const double vG  = 1. + .018 * t913 + 5.249 * t923 + .650 * t9 + 19.176 * t943 + 6.034 * t953;
+            const double dvA = 24.811 * t9i2;
+            const double dvB = 1.0663 * t9i2;
+            const double dvC = t9i43 * 13.49 / 3. - 208.246564 * t9;
+            const double dvD = (t9i23 * .031 + t9i13 * 16.018 + 5.196 + t913 * 199.532 + t923 * 137.13) / 3.;
+            const double dvE = 3.3364 * t9i2;
+            const double dvF = t9i43 * 23.57 / 3. - 36.21548212 * t9;
+            const double dvG = (t9i23 * .018 + t9i13 * 10.498 + 1.950 + t913 * 76.704 + t923 * 30.17) / 3.;
+
+            const double r2abe  = 7.4e5 * t9i32 * std::exp(vB) + 4.164e9 * t9i23 * vD * std::exp(vC);
+            const double rbeac  = 130. * t9i32 * std::exp(vE) + 2.510e7 * t9i23 * vG * std::exp(vF);
+            const double dr2abe = std::exp(vB) * (-1.11e6 * t9i52 + 7.4e5 * t9i32 * dvB) +
+                                  4.164e9 * std::exp(vC) * (-t9i53 * vD * 2. / 3. + t9i23 * dvC * vD + t9i23 * dvD);
+            const double drbeac = std::exp(vE) * (-195. * t9i52 + 130. * t9i32 * dvE) +
+                                  2.510e7 * std::exp(vF) * (-2. * t9i53 * vG / 3. + t9i23 * dvF * vG + t9i23 * dvG);
+            o.dr3a = (2.90e-16 * (dr2abe * rbeac + r2abe * drbeac) + 1.35e-8 * std::exp(vA) * (-1.5 * t9i52 + t9i32 * dvA)) *
+                     1.e-9;
+        }
+        // d(reverse triple alpha)/dT
+        {
+            const double vA  = -84.419412 * t9i;
+            const double dvA = 84.419412 * t9i2;
+            o.drg3a          = 2.00e20 * std::exp(vA) * t93 * (dvA * o.r3a + 3. * t9i * o.r3a + o.dr3a) * 1.e-9;
+        }
+        // d(12C+12C)/dT
+        {
+            const double vA   = t9 / (1. + .0396 * t9);
+            const double vA56 = std::pow(vA, 5. / 6.);
+            const double vB   = -84.165 * std::pow(vA, -1. / 3.) - 2.12e-3 * t93;
+            const double dvA  = vA * vA * t9i2;
+            const double dvB  = 28.055 * dvA * std::pow(vA, -4. / 3.) - 6.36e-3 * t92;
+            o.dr24 = 4.27e26 * t9i32 * std::exp(vB) * (std::pow(vA, -1. / 6.) * dvA * 5. / 6. - 1.5 * vA56 * t9i + vA56 * dvB) *
+                     1.e-9;
+        }
+        // d(12C+16O)/dT
+        if (t9 > .5)
+        {
+            const double vA   = t9 / (1. + .055 * t9);
+            const double vA56 = std::pow(vA, 5. / 6.);
+            const double vB   = -106.594 * std::pow(vA, -1. / 3.);
+            const double vC   = -.18 * vA * vA;
+            const double vD   = 2.562 * std::pow(vA, 2. / 3.);
+            const double val  = std::exp(vC) + 1.06e-3 * std::exp(vD);
+            const double dvA  = vA * vA * t9i2;
+            const double dvB  = 106.594 * std::pow(vA, -4. / 3.) * dvA / 3.;
+            const double dvC  = -.36 * vA * dvA;
+            const double dvD  = 1.708 * dvA * std::pow(vA, -1. / 3.);
+            const double dval = dvC * std::exp(vC) + 1.06e-3 * dvD * std::exp(vD);
+            o.dr1216          = 1.72e31 * t9i32 *
+                       (std::pow(vA, -1. / 6.) * dvA * 5. / 6. - 1.5 * vA56 * t9i + vA56 * (dvB - dval / val)) *
+                       std::exp(vB) / val * 1.e-9;
+        }
+        // d(16O+16O)/dT
+        {
+            const double vA  = -135.93e0 * t9i13 - .629 * t923 - .445 * t943 + .0103 * t92;
+            const double dvA = 45.31 * t9i43 - .629 * t9i13 * 2. / 3. - .445 * t913 * 4. / 3. + .0206 * t9;
+            o.dr32           = 7.10e36 * std::exp(vA) * t9i23 * (-t9i * 2. / 3. + dvA) * 1.e-9;
+        }
+        // d(12C(a,g))/dT
+        {
+            const double vB  = std::pow(1. + .0489 * t9i23, 2);
+            const double vC  = -32.120 * t9i13 - std::pow(t9 / 3.496, 2.);
+            const double vD  = std::pow(1. + .2654 * t9i23, 2);
+            const double vE  = -32.120 * t9i13;
+            const double vF  = -27.499 * t9i;
+            const double vG  = -15.541 * t9i;
+            const double dvB = -(2. + .0978 * t9i23) * (.0326 * t9i53);
+            const double dvC = 32.120 * t9i43 / 3. - 2. * t9 / std::pow(3.496, 2.);
+            const double dvD = -(2. + .5308 * t9i23) * (.5308 * t9i53 / 3.);
+            const double dvE = 32.120 * t9i43 / 3.;
+            const double dvF = 27.499 * t9i2;
+            const double dvG = 15.541 * t9i2;
+            o.drcag          = (1.04e8 * std::exp(vC) * t9i2 * (-2. * t9i + dvC - dvB / vB) / vB +
+                       1.76e8 * std::exp(vE) * t9i2 * (-2. * t9i + dvE - dvD / vD) / vD +
+                       1.25e3 * std::exp(vF) * (-1.5 * t9i52 + dvF * t9i32) +
+                       1.43e-2 * std::exp(vG) * (5. * t94 + dvG * t95)) *
+                      1.e-9;
+        }
+        // d(reverse 12C(a,g))/dT and d(16O(a,g))/dT
+        {
+            const double vA  = -83.108047 * t9i;
+            const double vB  = -39.757 * t9i13 - std::pow(t9 / 1.586, 2.);
+            const double vC  = -10.297 * t9i;
+            const double vD  = -12.226 * t9i;
+            const double vE  = -20.093 * t9i;
+            const double dvA = 83.108047 * t9i2;
+            const double dvB = 39.757 * t9i43 / 3. - 2. * t9 / std::pow(1.586, 2.);
+            const double dvC = 10.297 * t9i2;
+            const double dvD = 12.226 * t9i2;
+            const double dvE = 20.093 * t9i2;
+            o.droga = 5.13e10 * std::exp(vA) * (o.drcag * t932 + o.rcag * 1.5 * t912 + o.rcag * t932 * dvA) * 1.e-9;
+            o.droag = (9.37e9 * std::exp(vB) * (-t9i53 * 2. / 3. + t9i23 * dvB) +
+                       62.1 * std::exp(vC) * (-1.5 * t9i52 + t9i32 * dvC) +
+                       538. * std::exp(vD) * (-1.5 * t9i52 + t9i32 * dvD) + 13. * std::exp(vE) * (2. * t9 + t92 * dvE)) *
+                      1.e-9;
+        }
+        // d(reverse 16O(a,g))/dT
+        {
+            const double vA  = -54.903255 * t9i;
+            const double dvA = 54.903255 * t9i2;
+            o.drnega = 5.65e10 * std::exp(vA) * (o.droag * t932 + 1.5 * o.roag * t912 + o.roag * t932 * dvA) * 1.e-9;
+        }
+    }
+    return o;
+}
+
+// Fitted rate exponent and its T-derivative (net14.cpp:116-130, net86.cpp:115-129).
+struct FitPowers
+{
+    double t9, t913, t923, t953, t9i, t9i2, t9i13, t9i23, t9i43, lt9;
+};
+inline FitPowers fit_powers(double T)
+{
+    FitPowers p;
+    p.t9    = T / 1.e9;
+    p.t913  = std::pow(p.t9, 1. / 3.);
+    p.t923  = p.t913 * p.t913;
+    p.t953  = std::pow(p.t9, 5. / 3.);
+    p.t9i   = 1.e0 / p.t9;
+    p.t9i2  = p.t9i * p.t9i;
+    p.t9i13 = 1.e0 / p.t913;
+    p.t9i23 = p.t9i13 * p.t9i13;
+    p.t9i43 = p.t9i23 * p.t9i23;
+    p.lt9   = std::log(p.t9);
+    return p;
+}
+inline void fit_exponent(const double* f, const FitPowers& p, double& coef, double& dcoef)
+{
+    coef  = f[1] * p.t9i + f[2] * p.t9i13 + f[3] * p.t913 + f[4] * p.t9 + f[5] * p.t953 + f[6] * p.lt9;
+    dcoef = (-f[1] * p.t9i2 + (-f[2] * p.t9i43 + f[3] * p.t9i23 + 5 * f[5] * p.t923) * (1. / 3.) + f[4] + f[6] * p.t9i) *
+            1e-9;
+}
+
+// Coulomb screening factor applied to a rate and its derivative (net14.cpp:628-632).
+inline void screen(double dmu, double T, double& r, double& dr)
+{
+    const double EF = std::exp(dmu);
+    r               = r * EF;
+    dr              = dr * EF - 2. * r * dmu / T;
+}
+
+// ------------------------------------------------------------------------------------------------
+// net14: 14 species, 29 reactions (net14.hpp:69-134; rates net14.cpp:40-660)
+// ------------------------------------------------------------------------------------------------
+inline void rates_net14(double T, double rho, double* BE, double* rates, double* drates)
+{
+    double BE0[14];
+    for (int s = 0; s < 14; ++s)
+        BE0[s] = NNET14_BE_MEV[s] * phys::MevToErg;
+    corrected_binding_energies(14, BE0, NNET14_Z, T, rho, BE);
+
+    double coefs[10], dcoefs[10], eff[16] = {0.}, deff[16] = {0.}, l[16] = {0.}, dl[16] = {0.}, mukbt[16] = {0.},
+                                  dmu[16] = {0.};
+    const FitPowers p = fit_powers(T);
+    for (int i = 4; i < 14; ++i)
+    {
+        fit_exponent(NNET14_FIT[i - 4], p, coefs[i - 4], dcoefs[i - 4]);
+        eff[i - 1]  = std::exp(NNET14_FIT[i - 4][0] + coefs[i - 4]);
+        deff[i - 1] = eff[i - 1] * dcoefs[i - 4];
+    }
+    {
+        const double t9   = T / 1.e9;
+        const double t9i  = 1.e0 / t9;
+        const double lt9  = std::log(t9);
+        const double val1 = 11.6045e0 * t9i;
+        const double val2 = 1.5e0 * lt9;
+        const double val3 = val1 * t9i * 1e-9;
+        const double val4 = 1.5e-9 * t9i;
+        const int    k    = temperature_bin(T);
+        for (int i = 4; i < 14; ++i)
+        {
+            l[i - 1] = NNET14_CHOOSE[i - 4][k] / NNET14_CHOOSE[i + 1 - 4][k] *
+                       std::exp(coefs[i - 4] + NNET14_FIT[i - 4][7] - NNET14_Q[i - 4] * val1 + val2);
+            dl[i - 1] = l[i - 1] * (dcoefs[i - 4] + NNET14_Q[i - 4] * val3 + val4);
+        }
+    }
+    const SpecialRates s = special_rates(T);
+    eff[0]               = s.r3a;
+    l[0]                 = s.rg3a;
+    eff[13]              = s.r24;
+    eff[14]              = s.r1216;
+    eff[15]              = s.r32;
+    eff[1]               = s.rcag;
+    l[1]                 = s.roga;
+    eff[2]               = s.roag;
+    l[2]                 = s.rnega;
+    deff[0]              = s.dr3a;
+    dl[0]                = s.drg3a;
+    deff[13]             = s.dr24;
+    if (T * 1.0e-09 > .5) deff[14] = s.dr1216;
+    deff[15] = s.dr32;
+    deff[1]  = s.drcag;
+    dl[1]    = s.droga;
+    deff[2]  = s.droag;
+    dl[2]    = s.drnega;
+
+    // screening of the forward rates, net14.cpp:572-641
+    screening_mu(14, NNET14_Z, T, rho, mukbt);
+    for (int i = 0; i < 14; ++i)
+        dmu[i] = mukbt[i] + mukbt[0] - mukbt[i + 1];
+    dmu[0] += mukbt[0];
+    dmu[13] = 2. * mukbt[1] - mukbt[3];
+    dmu[14] = mukbt[1] + mukbt[2] - mukbt[4];
+    dmu[15] = 2. * mukbt[2] - mukbt[5];
+    for (int i = 0; i < 16; ++i)
+        screen(dmu[i], T, eff[i], deff[i]);
+
+    // rate order of net14::reactionList, net14.cpp:646-659
+    int idx = 0;
+    for (int i = 0; i < 16; ++i, ++idx)
+    {
+        rates[idx]  = eff[i];
+        drates[idx] = deff[i];
+    }
+    for (int i = 0; i < 13; ++i, ++idx)
+    {
+        rates[idx]  = l[i];
+        drates[idx] = dl[i];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// net86: 86 species, 309 reactions (net86.hpp:95-192; rates net86.cpp:41-686)
+// ------------------------------------------------------------------------------------------------
+inline void rates_net86(double T, double rho, double* BE, double* rates, double* drates)
+{
+    double BE0[86];
+    for (int s = 0; s < 86; ++s)
+        BE0[s] = NNET86_BE_MEV[s] * phys::MevToErg;
+    corrected_binding_energies(86, BE0, NNET86_Z, T, rho, BE);
+
+    double coefs[150], dcoefs[150], eff[157] = {0.}, deff[157] = {0.}, l[157] = {0.}, dl[157] = {0.}, mukbt[86] = {0.},
+                                    dmu[157] = {0.};
+    const FitPowers p = fit_powers(T);
+    for (int i = 7; i < 157; ++i)
+    {
+        fit_exponent(NNET86_FIT[i - 7], p, coefs[i - 7], dcoefs[i - 7]);
+        eff[i]  = std::exp(NNET86_FIT[i - 7][0] + coefs[i - 7]);
+        deff[i] = eff[i] * dcoefs[i - 7];
+    }
+    const SpecialRates s = special_rates(T);
+    eff[0]               = s.r24;
+    eff[1]               = s.r1216;
+    eff[2]               = s.r32;
+    eff[4]               = s.r3a;
+    l[4]                 = s.rg3a;
+    eff[5]               = s.rcag;
+    l[5]                 = s.roga;
+    eff[6]               = s.roag;
+    l[6]                 = s.rnega;
+    deff[4]              = s.dr3a;
+    dl[4]                = s.drg3a;
+    deff[0]              = s.dr24;
+    if (T * 1.0e-09 > .5) deff[1] = s.dr1216;
+    deff[2] = s.dr32;
+    deff[5] = s.drcag;
+    dl[5]   = s.droga;
+    deff[6] = s.droag;
+    dl[6]   = s.drnega;
+
+    // reverse rates from partition-function ratios, net86.cpp:524-554
+    {
+        const double t9   = T * 1e-09;
+        const double t9i  = 1. / t9;
+        const double val1 = 11.6045 * t9i;
+        const double val2 = 1.5 * std::log(t9);
+        const double val3 = val1 * t9i * 1e-9;
+        const double val4 = 1.5e-9 * t9i;
+        const int    k    = temperature_bin(T);
+        for (int i = 7; i < 157; ++i)
+        {
+            const double part =
+                NNET86_CHOOSE[NNET86_MAIN_REACTANT[i] - 5][k] / NNET86_CHOOSE[NNET86_MAIN_PRODUCT[i] - 5][k];
+            if (i < 137)
+            {
+                l[i]  = part * std::exp(NNET86_FIT[i - 7][7] + coefs[i - 7] - val1 * NNET86_Q[i - 7] + val2);
+                dl[i] = l[i] * (dcoefs[i - 7] + val3 * NNET86_Q[i - 7] + val4);
+            }
+            else
+            {
+                l[i]  = part * std::exp(NNET86_FIT[i - 7][7] + coefs[i - 7] - val1 * NNET86_Q[i - 7]);
+                dl[i] = l[i] * (dcoefs[i - 7] + val3 * NNET86_Q[i - 7]);
+            }
+        }
+    }
+    // (a,p)/(a,n): the tabulated direction is the network's reverse, net86.cpp:561-567
+    for (int i = 137; i < 157; ++i)
+        if (i < 154 || i == 156)
+        {
+            std::swap(eff[i], l[i]);
+            std::swap(deff[i], dl[i]);
+        }
+
+    // screening, net86.cpp:572-659
+    screening_mu(86, NNET86_Z, T, rho, mukbt);
+    mukbt[1] = 0; // neutron
+    for (int i = 0; i < 157; ++i)
+        dmu[i] = mukbt[NNET86_MAIN_REACTANT[i]] + mukbt[NNET86_SECONDARY_REACTANT[i]] - mukbt[NNET86_MAIN_PRODUCT[i]];
+    dmu[4] += mukbt[NNET86_MAIN_REACTANT[4]];
+    for (int i = 0; i < 157; ++i)
+        screen(dmu[i], T, eff[i], deff[i]);
+    for (int i = 137; i < 157; ++i)
+        screen(dmu[i], T, l[i], dl[i]);
+
+    // rate order of net86::reactionList, net86.cpp:664-685
+    int idx = 0;
+    for (int i = 0; i < 157; ++i)
+        if (i != 3)
+        {
+            rates[idx]  = eff[i];
+            drates[idx] = deff[i];
+            ++idx;
+        }
+    for (int i = 4; i < 157; ++i, ++idx)
+    {
+        rates[idx]  = l[i];
+        drates[idx] = dl[i];
+    }
+}
+
+// net87.cpp:102-141: bilinear interpolation in (log10 T, log10 rho*Ye) of the 12 electron
+// capture quantities.
+inline void electron_interpolate(double temp, double rhoElec, double* rate)
+{
+    const int nT = NNET_ELEC_NT, nR = NNET_ELEC_NR, nC = NNET_ELEC_NC;
+    int       it = 0;
+    double    lt = std::log10(temp);
+    while (it < nT && NNET_ELEC_LOGT[it] < lt)
+        ++it;
+    int    ir = 0;
+    double lr = std::log10(rhoElec);
+    while (ir < nR && NNET_ELEC_LOGRHO[ir] < lr)
+        ++ir;
+    const int it0 = std::max(0, it - 1), ir0 = std::max(0, ir - 1);
+    it            = std::min(nT - 1, it);
+    ir            = std::min(nR - 1, ir);
+    const double x2x  = NNET_ELEC_LOGT[it] - lt;
+    const double xx1  = -NNET_ELEC_LOGT[it0] + lt;
+    const double y2y  = NNET_ELEC_LOGRHO[ir] - lr;
+    const double yy1  = -NNET_ELEC_LOGRHO[ir0] + lr;
+    double       x2x1 = NNET_ELEC_LOGT[it] - NNET_ELEC_LOGT[it0];
+    x2x1              = x2x1 == 0 ? 2 : x2x1;
+    double y2y1       = NNET_ELEC_LOGRHO[ir] - NNET_ELEC_LOGRHO[ir0];
+    y2y1              = y2y1 == 0 ? 2 : y2y1;
+    auto E            = [&](int a, int b, int c) { return NNET_ELEC_RATE[(a * nR + b) * nC + c]; };
+    for (int c = 0; c < nC; ++c)
+        rate[c] = (E(it0, ir0, c) * x2x * y2y + E(it, ir0, c) * xx1 * y2y + E(it0, ir, c) * x2x * yy1 +
+                   E(it, ir, c) * xx1 * yy1) /
+                  (x2x1 * y2y1);
+}
+
+// net87: net86 + electron captures (net87.hpp:117-175).
+inline void rates_net87(const double* Y, double T, double rho, double* BE, double* rates, double* drates)
+{
+    const double Yelec   = Y[86];
+    const double rhoElec = Yelec * rho;
+    double       ev[12];
+    electron_interpolate(T, rhoElec, ev);
+    rates_net86(T, rho, BE, rates, drates);
+    const double kbt        = phys::Kb * T;
+    const double nakbt      = phys::Na * kbt;
+    const double correction = -1.5 * nakbt;
+    BE[86]                  = NNET87_ELECTRON_BE_MEV[0] * phys::MevToErg + correction;
+    BE[0] += -ev[3];
+    BE[1] += -ev[9];
+    rates[309]  = rhoElec == 0 ? 0 : ev[0] / rhoElec;
+    drates[309] = rhoElec == 0 ? 0 : ev[1] / rhoElec;
+    rates[310]  = rhoElec == 0 ? 0 : ev[6] / rhoElec;
+    drates[310] = rhoElec == 0 ? 0 : ev[7] / rhoElec;
+}
+
+} // namespace oracle
