@@ -1,0 +1,127 @@
+/* C-ABI of libnnet_mi355x.so: the MI355X (gfx950) implementation of SPH-EXA-nnet's batched
+ * implicit nuclear-network solve.
+ *
+ * It replaces the device half of
+ *   template<bool use_gpu, class Data, typename Float, class nseFunction>
+ *   void nnet::parallel::computeNuclearReactions(Data& n, size_t firstIndex, size_t lastIndex,
+ *       Float hydro_dt, Float previous_dt, const nnet::ReactionList& reactions,
+ *       const nnet::ComputeReactionRatesFunctor<Float>& construct_rates_BE,
+ *       const nnet::EosFunctor<Float>& eos, bool use_drhodt, nseFunction jumpToNse)
+ *                                     (include/nnet/parallel/parallel_nuclear_net.hpp:86-172)
+ * i.e. the reference's nuclear_net.cuh moveToGpu (:61-72) + cudaComputeNuclearReactions
+ * (parallel/CUDA/parallel_nuclear_net.cu:116-163).  Plain pointers and sizes only; all particle
+ * arrays are caller-owned device memory in the reference's structure-of-arrays layout.
+ *
+ * Every entry point returns 0 on success or a negative NNET_E* code; nnet_last_error() gives the
+ * message.  Nothing throws across this boundary.
+ */
+#ifndef NNET_MI355X_H
+#define NNET_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define NNET_OK 0
+#define NNET_E_ARG (-1)       /* invalid argument (null pointer, size, unknown kind) */
+#define NNET_E_HIP (-2)       /* HIP runtime error (allocation, launch, copy) */
+#define NNET_E_TABLE (-3)     /* Helmholtz table missing or malformed */
+#define NNET_E_NETWORK (-4)   /* reaction list inconsistent with the rate parameterization */
+#define NNET_E_STATE (-5)     /* context not ready for this call (e.g. no Helmholtz table loaded) */
+
+/* rate parameterizations (construct_rates_BE functor types):
+ * nnet::net14/net86/net87::ComputeReactionRatesFunctor<double> */
+#define NNET_RATES_NET14 14
+#define NNET_RATES_NET86 86
+#define NNET_RATES_NET87 87
+
+/* EOS functor types: nnet::eos::IdealGasFunctor<double>, nnet::eos::HelmholtzFunctor<double> */
+#define NNET_EOS_IDEAL 0
+#define NNET_EOS_HELMHOLTZ 1
+
+/* arithmetic of the Newton linear solve */
+#define NNET_PRECISION_FP64 0  /* plain FP64 everywhere (parity path) */
+#define NNET_PRECISION_MIXED 1 /* FP32 LU + FP64 iterative refinement */
+
+/* per-particle status written to nnet_batch.status */
+#define NNET_STATUS_OK 0      /* integrated to hydro_dt */
+#define NNET_STATUS_SKIPPED 1 /* rho <= 1e5 or T <= 1e8: untouched (parallel_nuclear_net_impl.cpp:83) */
+#define NNET_STATUS_STALLED 2 /* the reference controller would replay a rejected step forever
+                                 (nuclear_net.hpp:744-757, 899-904); state left at the last
+                                 accepted substep */
+#define NNET_STATUS_ZERO_DT 3 /* substep of length 0 (the reference throws, nuclear_net.hpp:533-548) */
+#define NNET_STATUS_MAX_ITER 4 /* Newton-iteration cap reached (guard; the reference has none) */
+
+typedef struct nnet_ctx nnet_ctx;
+typedef struct nnet_network nnet_network;
+
+/* Context on one HIP device.  Replaces the reference's per-call cudaMalloc/cudaMemcpy of
+ * tables and functors (helmholtz.cpp:174-218, net87.cpp:89-99): constant tables are uploaded
+ * once here.  helm_table_path may be NULL (ideal gas only) or the path of a binary Helmholtz
+ * table (see nnet_helm_table_write / the package's table generator). */
+int nnet_ctx_create(nnet_ctx** ctx, int device, const char* helm_table_path);
+void nnet_ctx_destroy(nnet_ctx* ctx);
+const char* nnet_last_error(const nnet_ctx* ctx);
+
+/* Load (or replace) the Helmholtz table: 21 arrays of 541x201 doubles in the reference order
+ * f, fd, ft, fdd, ftt, fdt, fddt, fdtt, fddtt, dpdf, dpdfd, dpdft, dpdfdt, ef, efd, eft, efdt,
+ * xf, xfd, xft, xfdt, each [density][temperature] (helmholtz.cpp:100-137). */
+int nnet_ctx_load_helmholtz(nnet_ctx* ctx, const char* path);
+
+/* Network = reaction list + rate parameterization.  The list uses the reference's flat
+ * ReactionList layout (nuclear_net.hpp:168-213): reaction r has reactants
+ * [reactant_begin[r], product_begin[r]) and products [product_begin[r], reactant_begin[r+1])
+ * of (species_id, count).  reactant_begin has n_reactions+1 entries.  The order must be the
+ * rate functor's push order (as in net14::reactionList etc.). */
+int nnet_network_create(nnet_ctx* ctx, int rates_kind, int n_species, int n_reactions,
+                        const int* reactant_begin, const int* product_begin,
+                        const int* species_id, const int* count, nnet_network** net);
+/* The library's own copy of net14::reactionList / net86::reactionList / net87::reactionList. */
+int nnet_network_builtin(nnet_ctx* ctx, int rates_kind, nnet_network** net);
+void nnet_network_destroy(nnet_network* net);
+/* Copy out a network's reaction list (host buffers sized by the caller; NULL to query sizes). */
+int nnet_network_reactions(const nnet_network* net, int* n_reactions, int* n_entries,
+                           int* reactant_begin, int* product_begin, int* species_id, int* count);
+
+/* One batch of particles sharing a network and an EOS. */
+typedef struct nnet_batch {
+    const nnet_network* network;
+    int eos;              /* NNET_EOS_* */
+    double mu;            /* IdealGasFunctor(mu) */
+    const double* Z;      /* HelmholtzFunctor(Z, z_dim): host array of z_dim charges */
+    int z_dim;
+    int precision;        /* NNET_PRECISION_* */
+    size_t n;             /* particles [0, n) of the arrays below */
+    const double* rho;    /* device, n */
+    double* rho_m1;       /* device, n (read/written only if use_drhodt) */
+    double* const* Y;     /* host array of n_species device pointers, each n molar abundances */
+    double* temp;         /* device, n (in/out) */
+    double* dt;           /* device, n: nuclear timestep, persistent solver state (in/out) */
+    uint8_t* status;      /* device, n, optional (NULL) */
+    uint32_t* substeps;   /* device, n, optional: accepted substeps this call */
+    uint32_t* newton;     /* device, n, optional: Newton iterations this call */
+} nnet_batch;
+
+/* computeNuclearReactions for nb batches (different networks may share one call).
+ * Stream-ordered on `stream` (hipStream_t, NULL = default stream); returns after enqueueing.
+ * Semantics follow the reference CPU path (parallel_nuclear_net_impl.cpp:63-103): particles with
+ * rho <= 1e5 or T <= 1e8 are skipped; with use_drhodt, rho_m1[i] (if nonzero) is overwritten by
+ * (rho - rho_m1)/previous_dt and the integration uses drho/dt = 0. */
+int nnet_compute(nnet_ctx* ctx, const nnet_batch* batches, int nb, double hydro_dt,
+                 double previous_dt, int use_drhodt, void* stream);
+
+/* Cap on Newton iterations per particle per call (default 200000; 0 = no cap). */
+int nnet_ctx_set_max_iterations(nnet_ctx* ctx, long max_iter);
+
+/* Blocking variant of nnet_compute (drop-in for the reference's synchronous launch). */
+int nnet_compute_sync(nnet_ctx* ctx, const nnet_batch* batches, int nb, double hydro_dt,
+                      double previous_dt, int use_drhodt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -632,4 +632,8 @@ extern "C"
     }
 
     int oracle_dynamic_batch_size(size_t n, int p) { return dynamic_batch_size(n, p); }
+
+    // Test-only sensitivity probe: 1 moves a deterministic 3/8 of the rate functions'
+    // exp/log/pow results by one ulp; 0 restores the exact restatement.
+    void oracle_set_ulp_perturbation(int on) { ulp_perturbation() = on; }
 }

@@ -15,10 +15,36 @@
 #include <algorithm>
 #include <cmath>
 
+#include <cstdint>
+#include <cstring>
+
 #include "../sph-exa-nnet_amd/csrc/nnet_tables.h"
 
 namespace oracle
 {
+// Sensitivity probe (test-only).  When enabled, a deterministic 3/8 of the exp/log/pow results
+// of the rate functions are moved by one ulp, emulating a libm whose results differ from glibc's
+// in the last bit.  Used to measure how far the reference dynamics amplify last-ulp differences,
+// which bounds the parity any implementation not bit-identical to glibc can reach.
+inline int& ulp_perturbation()
+{
+    static int on = 0;
+    return on;
+}
+inline double ulp_bump(double y, double key)
+{
+    if (!ulp_perturbation()) return y;
+    uint64_t u;
+    std::memcpy(&u, &key, 8);
+    u ^= u >> 29;
+    u *= 0x9E3779B97F4A7C15ull;
+    u ^= u >> 32;
+    if ((u & 7) < 3) return std::nextafter(y, (u & 8) ? INFINITY : -INFINITY);
+    return y;
+}
+inline double xexp(double x) { return ulp_bump(std::exp(x), x); }
+inline double xlog(double x) { return ulp_bump(std::log(x), x); }
+inline double xpow(double x, double y) { return ulp_bump(std::pow(x, y), x * 1.37 + y); }
 namespace phys
 {
 // parameterization/net14/net14_constants.hpp:45-50 (net86_constants.hpp:51-56 are identical)
@@ -51,7 +77,7 @@ inline double coulomb_strong(double x)
 inline double coulomb_weak(double x)
 {
     const double a = -.5 * std::sqrt(3.), b = .29561, c = 1.9885;
-    return a * x * std::sqrt(x) + b * std::pow(x, c);
+    return a * x * std::sqrt(x) + b * xpow(x, c);
 }
 
 // Temperature-dependent corrected binding energies, net14.cpp:49-75 / net86.cpp:50-73.
@@ -63,11 +89,11 @@ inline void corrected_binding_energies(int ns, const double* BE0, const double* 
     for (int s = 0; s < ns; ++s)
         BE[s] = BE0[s] + correction;
     const double ne  = rho * phys::Na / 2.;
-    const double ae  = std::pow((3. / 4.) / (phys::pi * ne), 1. / 3.);
+    const double ae  = xpow((3. / 4.) / (phys::pi * ne), 1. / 3.);
     const double gam = phys::e2 / (kbt * ae);
     for (int s = 0; s < ns; ++s)
     {
-        const double g = gam * std::pow(Z[s], 5. / 3.);
+        const double g = gam * xpow(Z[s], 5. / 3.);
         const double f = g > 1 ? coulomb_strong(g) : coulomb_weak(g);
         BE[s] -= nakbt * f;
     }
@@ -78,18 +104,18 @@ inline void screening_mu(int ns, const double* Z, double T, double rho, double* 
 {
     const double ne  = rho * phys::Na / 2.;
     const double kbt = T * phys::Kb;
-    const double ae  = std::pow(3. / (4. * phys::pi * ne), 1. / 3.);
+    const double ae  = xpow(3. / (4. * phys::pi * ne), 1. / 3.);
     const double gam = phys::e2 / (kbt * ae);
     const double a1 = -.898004, b1 = .96786, c1 = .220703, d1 = -.86097, e1 = 2.520058332;
     for (int s = 0; s < ns; ++s)
     {
-        const double gp  = gam * std::pow(Z[s], 5. / 3.);
+        const double gp  = gam * xpow(Z[s], 5. / 3.);
         const double sq  = std::sqrt(gp);
         const double sq2 = std::sqrt(sq);
         if (gp <= 1)
-            mukbt[s] = -(1. / std::sqrt(3.)) * gp * sq + std::pow(gp, 1.9885) * .29561 / 1.9885;
+            mukbt[s] = -(1. / std::sqrt(3.)) * gp * sq + xpow(gp, 1.9885) * .29561 / 1.9885;
         else
-            mukbt[s] = a1 * gp + 4. * (b1 * sq2 - c1 / sq2) + d1 * std::log(gp) - e1;
+            mukbt[s] = a1 * gp + 4. * (b1 * sq2 - c1 / sq2) + d1 * xlog(gp) - e1;
     }
 }
 
@@ -113,7 +139,7 @@ inline SpecialRates special_rates(double T)
         const double t93  = t92 * t9;
         const double t95  = t92 * t93;
         const double t912 = std::sqrt(t9);
-        const double t913 = std::pow(t9, 1. / 3.);
+        const double t913 = xpow(t9, 1. / 3.);
         const double t923 = t913 * t913;
         const double t932 = t9 * t912;
         const double t943 = t9 * t913;
@@ -127,28 +153,28 @@ inline SpecialRates special_rates(double T)
         const double t9r32 = t9 * std::sqrt(t9r);
 
         // triple alpha and its reverse
-        const double r2abe = (7.40e+05 * t9i32) * std::exp(-1.0663 * t9i) +
-                             4.164e+09 * t9i23 * std::exp(-13.49 * t9i13 - t92 / 0.009604) *
+        const double r2abe = (7.40e+05 * t9i32) * xexp(-1.0663 * t9i) +
+                             4.164e+09 * t9i23 * xexp(-13.49 * t9i13 - t92 / 0.009604) *
                                  (1. + 0.031 * t913 + 8.009 * t923 + 1.732 * t9 + 49.883 * t943 + 27.426 * t953);
-        const double rbeac = (130. * t9i32) * std::exp(-3.3364 * t9i) +
-                             2.510e+07 * t9i23 * std::exp(-23.57 * t9i13 - t92 / 0.055225) *
+        const double rbeac = (130. * t9i32) * xexp(-3.3364 * t9i) +
+                             2.510e+07 * t9i23 * xexp(-23.57 * t9i13 - t92 / 0.055225) *
                                  (1. + 0.018 * t913 + 5.249 * t923 + 0.650 * t9 + 19.176 * t943 + 6.034 * t953);
         if (T > 8e7)
-            o.r3a = 2.90e-16 * (r2abe * rbeac) + 0.1 * 1.35e-07 * t9i32 * std::exp(-24.811 * t9i);
+            o.r3a = 2.90e-16 * (r2abe * rbeac) + 0.1 * 1.35e-07 * t9i32 * xexp(-24.811 * t9i);
         else
             o.r3a = 2.90e-16 * (r2abe * rbeac) *
-                        (0.01 + 0.2 * (1. + 4. * std::exp(-std::pow(0.025 * t9i, 3.263))) /
-                                    (1. + 4. * std::exp(-std::pow(t9 / 0.025, 9.227)))) +
-                    0.1 * 1.35e-07 * t9i32 * std::exp(-24.811 * t9i);
-        const double rev = 2.e20 * std::exp(-84.419412e0 * t9i);
+                        (0.01 + 0.2 * (1. + 4. * xexp(-xpow(0.025 * t9i, 3.263))) /
+                                    (1. + 4. * xexp(-xpow(t9 / 0.025, 9.227)))) +
+                    0.1 * 1.35e-07 * t9i32 * xexp(-24.811 * t9i);
+        const double rev = 2.e20 * xexp(-84.419412e0 * t9i);
         o.rg3a           = o.r3a * rev * t93;
 
         // 12C + 12C
         {
             const double t9a   = t9 / (1. + 0.0396 * t9);
-            const double t9a13 = std::pow(t9a, 1. / 3.);
-            const double t9a56 = std::pow(t9a, 5. / 6.);
-            o.r24              = 4.27e+26 * t9a56 * t9i32 * std::exp(-84.165 / t9a13 - 2.12e-03 * t93);
+            const double t9a13 = xpow(t9a, 1. / 3.);
+            const double t9a56 = xpow(t9a, 5. / 6.);
+            o.r24              = 4.27e+26 * t9a56 * t9i32 * xexp(-84.165 / t9a13 - 2.12e-03 * t93);
         }
         // 12C + 16O
         o.r1216 = 0;
@@ -156,24 +182,24 @@ inline SpecialRates special_rates(double T)
         {
             const double t9ap    = t9 / (1. + 0.055 * t9);
             const double t9a2p   = t9ap * t9ap;
-            const double t9a13p  = std::pow(t9ap, 1. / 3.);
+            const double t9a13p  = xpow(t9ap, 1. / 3.);
             const double t9a23p  = t9a13p * t9a13p;
-            const double t9a56ap = std::pow(t9ap, 5. / 6.);
-            o.r1216              = 1.72e+31 * t9a56ap * t9i32 * std::exp(-106.594 / t9a13p) /
-                      (std::exp(-0.18 * t9a2p) + 1.06e-03 * std::exp(2.562 * t9a23p));
+            const double t9a56ap = xpow(t9ap, 5. / 6.);
+            o.r1216              = 1.72e+31 * t9a56ap * t9i32 * xexp(-106.594 / t9a13p) /
+                      (xexp(-0.18 * t9a2p) + 1.06e-03 * xexp(2.562 * t9a23p));
         }
         // 16O + 16O
-        o.r32 = 7.10e+36 * t9i23 * std::exp(-135.93 * t9i13 - 0.629 * t923 - 0.445 * t943 + 0.0103 * t92);
+        o.r32 = 7.10e+36 * t9i23 * xexp(-135.93 * t9i13 - 0.629 * t923 - 0.445 * t943 + 0.0103 * t92);
         // 12C(a,g)16O and reverse
-        o.rcag = 1.04e+08 / (t92 * std::pow(1. + 0.0489 * t9i23, 2.)) * std::exp(-32.120 * t9i13 - t92 / 12.222016) +
-                 1.76e+08 / (t92 * std::pow(1. + 0.2654 * t9i23, 2.)) * std::exp(-32.120 * t9i13) +
-                 1.25e+03 * t9i32 * std::exp(-27.499 * t9i) + 1.43e-02 * t95 * std::exp(-15.541 * t9i);
-        o.roga = o.rcag * 5.13e+10 * t9r32 * std::exp(-83.108047 * t9rm1);
+        o.rcag = 1.04e+08 / (t92 * std::pow(1. + 0.0489 * t9i23, 2.)) * xexp(-32.120 * t9i13 - t92 / 12.222016) +
+                 1.76e+08 / (t92 * std::pow(1. + 0.2654 * t9i23, 2.)) * xexp(-32.120 * t9i13) +
+                 1.25e+03 * t9i32 * xexp(-27.499 * t9i) + 1.43e-02 * t95 * xexp(-15.541 * t9i);
+        o.roga = o.rcag * 5.13e+10 * t9r32 * xexp(-83.108047 * t9rm1);
         // 16O(a,g)20Ne and reverse
-        o.roag  = (9.37e+09 * t9i23 * std::exp(-39.757 * t9i13 - t92 / 2.515396) +
-                  62.1 * t9i32 * std::exp(-10.297 * t9i) + 538. * t9i32 * std::exp(-12.226 * t9i) +
-                  13. * t92 * std::exp(-20.093 * t9i));
-        o.rnega = o.roag * 5.65e+10 * t9r32 * std::exp(-54.93807 * t9rm1);
+        o.roag  = (9.37e+09 * t9i23 * xexp(-39.757 * t9i13 - t92 / 2.515396) +
+                  62.1 * t9i32 * xexp(-10.297 * t9i) + 538. * t9i32 * xexp(-12.226 * t9i) +
+                  13. * t92 * xexp(-20.093 * t9i));
+        o.rnega = o.roag * 5.65e+10 * t9r32 * xexp(-54.93807 * t9rm1);
     }
     {
         const double t9    = T * 1.0e-09;
@@ -182,7 +208,7 @@ inline SpecialRates special_rates(double T)
         const double t94   = t93 * t9;
         const double t95   = t94 * t9;
         const double t912  = std::sqrt(t9);
-        const double t913  = std::pow(t9, 1. / 3.);
+        const double t913  = xpow(t9, 1. / 3.);
         const double t923  = t913 * t913;
         const double t932  = t9 * t912;
         const double t943  = t9 * t913;
@@ -214,60 +240,60 @@ inline SpecialRates special_rates(double T)
             const double dvF = t9i43 * 23.57 / 3. - 36.21548212 * t9;
             const double dvG = (t9i23 * .018 + t9i13 * 10.498 + 1.950 + t913 * 76.704 + t923 * 30.17) / 3.;
 
-            const double r2abe  = 7.4e5 * t9i32 * std::exp(vB) + 4.164e9 * t9i23 * vD * std::exp(vC);
-            const double rbeac  = 130. * t9i32 * std::exp(vE) + 2.510e7 * t9i23 * vG * std::exp(vF);
-            const double dr2abe = std::exp(vB) * (-1.11e6 * t9i52 + 7.4e5 * t9i32 * dvB) +
-                                  4.164e9 * std::exp(vC) * (-t9i53 * vD * 2. / 3. + t9i23 * dvC * vD + t9i23 * dvD);
-            const double drbeac = std::exp(vE) * (-195. * t9i52 + 130. * t9i32 * dvE) +
-                                  2.510e7 * std::exp(vF) * (-2. * t9i53 * vG / 3. + t9i23 * dvF * vG + t9i23 * dvG);
-            o.dr3a = (2.90e-16 * (dr2abe * rbeac + r2abe * drbeac) + 1.35e-8 * std::exp(vA) * (-1.5 * t9i52 + t9i32 * dvA)) *
+            const double r2abe  = 7.4e5 * t9i32 * xexp(vB) + 4.164e9 * t9i23 * vD * xexp(vC);
+            const double rbeac  = 130. * t9i32 * xexp(vE) + 2.510e7 * t9i23 * vG * xexp(vF);
+            const double dr2abe = xexp(vB) * (-1.11e6 * t9i52 + 7.4e5 * t9i32 * dvB) +
+                                  4.164e9 * xexp(vC) * (-t9i53 * vD * 2. / 3. + t9i23 * dvC * vD + t9i23 * dvD);
+            const double drbeac = xexp(vE) * (-195. * t9i52 + 130. * t9i32 * dvE) +
+                                  2.510e7 * xexp(vF) * (-2. * t9i53 * vG / 3. + t9i23 * dvF * vG + t9i23 * dvG);
+            o.dr3a = (2.90e-16 * (dr2abe * rbeac + r2abe * drbeac) + 1.35e-8 * xexp(vA) * (-1.5 * t9i52 + t9i32 * dvA)) *
                      1.e-9;
         }
         // d(reverse triple alpha)/dT
         {
             const double vA  = -84.419412 * t9i;
             const double dvA = 84.419412 * t9i2;
-            o.drg3a          = 2.00e20 * std::exp(vA) * t93 * (dvA * o.r3a + 3. * t9i * o.r3a + o.dr3a) * 1.e-9;
+            o.drg3a          = 2.00e20 * xexp(vA) * t93 * (dvA * o.r3a + 3. * t9i * o.r3a + o.dr3a) * 1.e-9;
         }
         // d(12C+12C)/dT
         {
             const double vA   = t9 / (1. + .0396 * t9);
-            const double vA56 = std::pow(vA, 5. / 6.);
-            const double vB   = -84.165 * std::pow(vA, -1. / 3.) - 2.12e-3 * t93;
+            const double vA56 = xpow(vA, 5. / 6.);
+            const double vB   = -84.165 * xpow(vA, -1. / 3.) - 2.12e-3 * t93;
             const double dvA  = vA * vA * t9i2;
-            const double dvB  = 28.055 * dvA * std::pow(vA, -4. / 3.) - 6.36e-3 * t92;
-            o.dr24 = 4.27e26 * t9i32 * std::exp(vB) * (std::pow(vA, -1. / 6.) * dvA * 5. / 6. - 1.5 * vA56 * t9i + vA56 * dvB) *
+            const double dvB  = 28.055 * dvA * xpow(vA, -4. / 3.) - 6.36e-3 * t92;
+            o.dr24 = 4.27e26 * t9i32 * xexp(vB) * (xpow(vA, -1. / 6.) * dvA * 5. / 6. - 1.5 * vA56 * t9i + vA56 * dvB) *
                      1.e-9;
         }
         // d(12C+16O)/dT
         if (t9 > .5)
         {
             const double vA   = t9 / (1. + .055 * t9);
-            const double vA56 = std::pow(vA, 5. / 6.);
-            const double vB   = -106.594 * std::pow(vA, -1. / 3.);
+            const double vA56 = xpow(vA, 5. / 6.);
+            const double vB   = -106.594 * xpow(vA, -1. / 3.);
             const double vC   = -.18 * vA * vA;
-            const double vD   = 2.562 * std::pow(vA, 2. / 3.);
-            const double val  = std::exp(vC) + 1.06e-3 * std::exp(vD);
+            const double vD   = 2.562 * xpow(vA, 2. / 3.);
+            const double val  = xexp(vC) + 1.06e-3 * xexp(vD);
             const double dvA  = vA * vA * t9i2;
-            const double dvB  = 106.594 * std::pow(vA, -4. / 3.) * dvA / 3.;
+            const double dvB  = 106.594 * xpow(vA, -4. / 3.) * dvA / 3.;
             const double dvC  = -.36 * vA * dvA;
-            const double dvD  = 1.708 * dvA * std::pow(vA, -1. / 3.);
-            const double dval = dvC * std::exp(vC) + 1.06e-3 * dvD * std::exp(vD);
+            const double dvD  = 1.708 * dvA * xpow(vA, -1. / 3.);
+            const double dval = dvC * xexp(vC) + 1.06e-3 * dvD * xexp(vD);
             o.dr1216          = 1.72e31 * t9i32 *
-                       (std::pow(vA, -1. / 6.) * dvA * 5. / 6. - 1.5 * vA56 * t9i + vA56 * (dvB - dval / val)) *
-                       std::exp(vB) / val * 1.e-9;
+                       (xpow(vA, -1. / 6.) * dvA * 5. / 6. - 1.5 * vA56 * t9i + vA56 * (dvB - dval / val)) *
+                       xexp(vB) / val * 1.e-9;
         }
         // d(16O+16O)/dT
         {
             const double vA  = -135.93e0 * t9i13 - .629 * t923 - .445 * t943 + .0103 * t92;
             const double dvA = 45.31 * t9i43 - .629 * t9i13 * 2. / 3. - .445 * t913 * 4. / 3. + .0206 * t9;
-            o.dr32           = 7.10e36 * std::exp(vA) * t9i23 * (-t9i * 2. / 3. + dvA) * 1.e-9;
+            o.dr32           = 7.10e36 * xexp(vA) * t9i23 * (-t9i * 2. / 3. + dvA) * 1.e-9;
         }
         // d(12C(a,g))/dT
         {
-            const double vB  = std::pow(1. + .0489 * t9i23, 2);
+            const double vB  = std::pow(1. + .0489 * t9i23, 2.);
             const double vC  = -32.120 * t9i13 - std::pow(t9 / 3.496, 2.);
-            const double vD  = std::pow(1. + .2654 * t9i23, 2);
+            const double vD  = std::pow(1. + .2654 * t9i23, 2.);
             const double vE  = -32.120 * t9i13;
             const double vF  = -27.499 * t9i;
             const double vG  = -15.541 * t9i;
@@ -277,10 +303,10 @@ inline SpecialRates special_rates(double T)
             const double dvE = 32.120 * t9i43 / 3.;
             const double dvF = 27.499 * t9i2;
             const double dvG = 15.541 * t9i2;
-            o.drcag          = (1.04e8 * std::exp(vC) * t9i2 * (-2. * t9i + dvC - dvB / vB) / vB +
-                       1.76e8 * std::exp(vE) * t9i2 * (-2. * t9i + dvE - dvD / vD) / vD +
-                       1.25e3 * std::exp(vF) * (-1.5 * t9i52 + dvF * t9i32) +
-                       1.43e-2 * std::exp(vG) * (5. * t94 + dvG * t95)) *
+            o.drcag          = (1.04e8 * xexp(vC) * t9i2 * (-2. * t9i + dvC - dvB / vB) / vB +
+                       1.76e8 * xexp(vE) * t9i2 * (-2. * t9i + dvE - dvD / vD) / vD +
+                       1.25e3 * xexp(vF) * (-1.5 * t9i52 + dvF * t9i32) +
+                       1.43e-2 * xexp(vG) * (5. * t94 + dvG * t95)) *
                       1.e-9;
         }
         // d(reverse 12C(a,g))/dT and d(16O(a,g))/dT
@@ -295,17 +321,17 @@ inline SpecialRates special_rates(double T)
             const double dvC = 10.297 * t9i2;
             const double dvD = 12.226 * t9i2;
             const double dvE = 20.093 * t9i2;
-            o.droga = 5.13e10 * std::exp(vA) * (o.drcag * t932 + o.rcag * 1.5 * t912 + o.rcag * t932 * dvA) * 1.e-9;
-            o.droag = (9.37e9 * std::exp(vB) * (-t9i53 * 2. / 3. + t9i23 * dvB) +
-                       62.1 * std::exp(vC) * (-1.5 * t9i52 + t9i32 * dvC) +
-                       538. * std::exp(vD) * (-1.5 * t9i52 + t9i32 * dvD) + 13. * std::exp(vE) * (2. * t9 + t92 * dvE)) *
+            o.droga = 5.13e10 * xexp(vA) * (o.drcag * t932 + o.rcag * 1.5 * t912 + o.rcag * t932 * dvA) * 1.e-9;
+            o.droag = (9.37e9 * xexp(vB) * (-t9i53 * 2. / 3. + t9i23 * dvB) +
+                       62.1 * xexp(vC) * (-1.5 * t9i52 + t9i32 * dvC) +
+                       538. * xexp(vD) * (-1.5 * t9i52 + t9i32 * dvD) + 13. * xexp(vE) * (2. * t9 + t92 * dvE)) *
                       1.e-9;
         }
         // d(reverse 16O(a,g))/dT
         {
             const double vA  = -54.903255 * t9i;
             const double dvA = 54.903255 * t9i2;
-            o.drnega = 5.65e10 * std::exp(vA) * (o.droag * t932 + 1.5 * o.roag * t912 + o.roag * t932 * dvA) * 1.e-9;
+            o.drnega = 5.65e10 * xexp(vA) * (o.droag * t932 + 1.5 * o.roag * t912 + o.roag * t932 * dvA) * 1.e-9;
         }
     }
     return o;
@@ -320,15 +346,15 @@ inline FitPowers fit_powers(double T)
 {
     FitPowers p;
     p.t9    = T / 1.e9;
-    p.t913  = std::pow(p.t9, 1. / 3.);
+    p.t913  = xpow(p.t9, 1. / 3.);
     p.t923  = p.t913 * p.t913;
-    p.t953  = std::pow(p.t9, 5. / 3.);
+    p.t953  = xpow(p.t9, 5. / 3.);
     p.t9i   = 1.e0 / p.t9;
     p.t9i2  = p.t9i * p.t9i;
     p.t9i13 = 1.e0 / p.t913;
     p.t9i23 = p.t9i13 * p.t9i13;
     p.t9i43 = p.t9i23 * p.t9i23;
-    p.lt9   = std::log(p.t9);
+    p.lt9   = xlog(p.t9);
     return p;
 }
 inline void fit_exponent(const double* f, const FitPowers& p, double& coef, double& dcoef)
@@ -341,7 +367,7 @@ inline void fit_exponent(const double* f, const FitPowers& p, double& coef, doub
 // Coulomb screening factor applied to a rate and its derivative (net14.cpp:628-632).
 inline void screen(double dmu, double T, double& r, double& dr)
 {
-    const double EF = std::exp(dmu);
+    const double EF = xexp(dmu);
     r               = r * EF;
     dr              = dr * EF - 2. * r * dmu / T;
 }
@@ -362,13 +388,13 @@ inline void rates_net14(double T, double rho, double* BE, double* rates, double*
     for (int i = 4; i < 14; ++i)
     {
         fit_exponent(NNET14_FIT[i - 4], p, coefs[i - 4], dcoefs[i - 4]);
-        eff[i - 1]  = std::exp(NNET14_FIT[i - 4][0] + coefs[i - 4]);
+        eff[i - 1]  = xexp(NNET14_FIT[i - 4][0] + coefs[i - 4]);
         deff[i - 1] = eff[i - 1] * dcoefs[i - 4];
     }
     {
         const double t9   = T / 1.e9;
         const double t9i  = 1.e0 / t9;
-        const double lt9  = std::log(t9);
+        const double lt9  = xlog(t9);
         const double val1 = 11.6045e0 * t9i;
         const double val2 = 1.5e0 * lt9;
         const double val3 = val1 * t9i * 1e-9;
@@ -377,7 +403,7 @@ inline void rates_net14(double T, double rho, double* BE, double* rates, double*
         for (int i = 4; i < 14; ++i)
         {
             l[i - 1] = NNET14_CHOOSE[i - 4][k] / NNET14_CHOOSE[i + 1 - 4][k] *
-                       std::exp(coefs[i - 4] + NNET14_FIT[i - 4][7] - NNET14_Q[i - 4] * val1 + val2);
+                       xexp(coefs[i - 4] + NNET14_FIT[i - 4][7] - NNET14_Q[i - 4] * val1 + val2);
             dl[i - 1] = l[i - 1] * (dcoefs[i - 4] + NNET14_Q[i - 4] * val3 + val4);
         }
     }
@@ -442,7 +468,7 @@ inline void rates_net86(double T, double rho, double* BE, double* rates, double*
     for (int i = 7; i < 157; ++i)
     {
         fit_exponent(NNET86_FIT[i - 7], p, coefs[i - 7], dcoefs[i - 7]);
-        eff[i]  = std::exp(NNET86_FIT[i - 7][0] + coefs[i - 7]);
+        eff[i]  = xexp(NNET86_FIT[i - 7][0] + coefs[i - 7]);
         deff[i] = eff[i] * dcoefs[i - 7];
     }
     const SpecialRates s = special_rates(T);
@@ -470,7 +496,7 @@ inline void rates_net86(double T, double rho, double* BE, double* rates, double*
         const double t9   = T * 1e-09;
         const double t9i  = 1. / t9;
         const double val1 = 11.6045 * t9i;
-        const double val2 = 1.5 * std::log(t9);
+        const double val2 = 1.5 * xlog(t9);
         const double val3 = val1 * t9i * 1e-9;
         const double val4 = 1.5e-9 * t9i;
         const int    k    = temperature_bin(T);
@@ -480,12 +506,12 @@ inline void rates_net86(double T, double rho, double* BE, double* rates, double*
                 NNET86_CHOOSE[NNET86_MAIN_REACTANT[i] - 5][k] / NNET86_CHOOSE[NNET86_MAIN_PRODUCT[i] - 5][k];
             if (i < 137)
             {
-                l[i]  = part * std::exp(NNET86_FIT[i - 7][7] + coefs[i - 7] - val1 * NNET86_Q[i - 7] + val2);
+                l[i]  = part * xexp(NNET86_FIT[i - 7][7] + coefs[i - 7] - val1 * NNET86_Q[i - 7] + val2);
                 dl[i] = l[i] * (dcoefs[i - 7] + val3 * NNET86_Q[i - 7] + val4);
             }
             else
             {
-                l[i]  = part * std::exp(NNET86_FIT[i - 7][7] + coefs[i - 7] - val1 * NNET86_Q[i - 7]);
+                l[i]  = part * xexp(NNET86_FIT[i - 7][7] + coefs[i - 7] - val1 * NNET86_Q[i - 7]);
                 dl[i] = l[i] * (dcoefs[i - 7] + val3 * NNET86_Q[i - 7]);
             }
         }

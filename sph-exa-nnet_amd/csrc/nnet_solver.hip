@@ -1,0 +1,1327 @@
+// libnnet_mi355x: MI355X (gfx950) implementation of SPH-EXA-nnet's batched implicit nuclear
+// network solve behind the C-ABI of include/nnet_mi355x.h.
+//
+// Execution model (FP64 path): one particle per workgroup, the workgroup's lanes share the work
+// of one Newton iteration:
+//   rates      one lane per fitted rate / hand-fitted group / species (nnet_dev_rates.hpp)
+//   assembly   the (d+1)x(d+1) Newton matrix is built directly into registers: lane (tr, tc) of
+//              a TR x TC lane grid owns rows tr + TR*a and columns tc + TC*b.  Each entry is
+//              accumulated from a host-precomputed ordered contribution list, so the value is
+//              the one the reference's scatter loops (nuclear_net.hpp:347-438) produce.
+//   LU         right-looking unpivoted elimination on the register-resident matrix with the
+//              reference's per-element update order (eigen.hpp:179-206); the normalised pivot
+//              rows are staged in LDS, which also serves the back substitution.
+//   controller the substep/Newton state machine of nuclear_net.hpp:692-959 is replicated in
+//              every lane (uniform control flow) with exact replay-stall detection.
+// A grid of one workgroup per particle lets the hardware dispatcher balance particles of very
+// different stiffness across the CUs.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/nnet_mi355x.h"
+#include "nnet_dev_rates.hpp"
+#include "nnet_dev_eos.hpp"
+#include "nnet_tables.h"
+
+namespace nnet_amd
+{
+// nuclear_net.hpp:48-106
+namespace k
+{
+constexpr double theta = 0.8, minTemp = 1e8, minRho = 1e5, maxDtStep = 2, minDtStep = 1e-2, nanDtStep = 2e-1;
+constexpr double epsilonSystem = 1e-40, epsilonVector = 1e-16;
+constexpr double NR_maxDt = 1e-2, NR_dTOverTempTarget = 1e-2, NR_dTOverTempTol = 4, NR_itTol = 1e-7;
+constexpr int    NR_maxIt      = 11;
+constexpr double substep_dtTol = 1e-6;
+} // namespace k
+
+constexpr int MAXQ = 3; // reactant entries per reaction
+
+// ---------------------------------------------------------------------------------------------
+// Per-network constant data in device memory (built on the host by build_network()).
+// ---------------------------------------------------------------------------------------------
+struct NetDev
+{
+    int net, d, nR, nE, nfit, fit_e0, rev_div;
+    SpecialIdx sp;
+    // rates
+    const double* BE0;     // [d'] constant binding energies (erg/mol), d' = 14 or 86
+    const double* Zr;      // [d'] charges used by the rate functor
+    const double* Z53;     // [d'] Z^(5/3) (host glibc pow, the value the reference computes)
+    const double* fit;     // [nfit][8]
+    const double* q;       // [nfit]
+    const double* choose;  // [rows][24]
+    const int*    ca;      // [nfit] choose row of the reverse numerator
+    const int*    cb;      // [nfit] choose row of the reverse denominator
+    const int*    photo;   // [nfit] reverse is a photodisintegration (carries T^3/2)
+    const int*    scr;     // [nE][4]: mu index a, b, c, e (e < 0: none)
+    const int*    swap;    // [nE]
+    const int*    scr_l;   // [nE] reverse rate is screened too
+    const int*    push_e;  // [nE] rate index of eff[i] (-1: not in the list)
+    const int*    push_l;  // [nE] rate index of l[i]
+    const double* elogT;   // net87 electron table
+    const double* elogR;
+    const double* etab;
+    double        BEe;     // net87 electron binding energy (erg/mol)
+    // reactions
+    const int* rb;   // [nR+1] reactant begin
+    const int* pb;   // [nR] product begin
+    const int* ids;  // species id of entry
+    const int* nums; // count of entry
+    // derivativesFromReactions: species s -> ordered (reaction, sign, count)
+    const int* sp_ptr;
+    const int* sp_r;
+    const int* sp_c; // signed count (+ produced / - consumed)
+    // firstOrderDYFromReactions: dense entry a*d+b -> ordered (reaction, reactant pos, signed count)
+    const int* jp_ptr;
+    const int* j_r;
+    const int* j_q;
+    const int* j_c;
+    // rows of the RHS correction: row a -> ascending b with a structural entry
+    const int* row_ptr;
+    const int* row_b;
+};
+
+constexpr int MAXSP = 88; // species + 1 of the largest network
+
+// Passed by value in the kernel arguments: no device allocation per call.
+struct EosDev
+{
+    int             kind;
+    int             zdim;
+    double          mu;
+    const HelmGrid* grid;
+    const double*   cells;
+    double          Z[MAXSP];
+};
+
+struct BatchDev
+{
+    size_t    n;
+    const double* rho;
+    double*   rho_m1;
+    double*   temp;
+    double*   dt;
+    uint8_t*  status;
+    uint32_t* substeps;
+    uint32_t* newton;
+    double*   Y[MAXSP]; // per-species abundance arrays (SoA)
+};
+
+struct StepParams
+{
+    double hydro_dt, previous_dt;
+    int    use_drhodt;
+    long   max_iter;
+};
+
+// x^n for the stoichiometric counts 0..3 (std::pow(x, n) with n integral):
+// n = 3 is the correctly rounded cube (double-double product), as glibc's pow returns.
+__device__ __forceinline__ double pow_count(double x, int n)
+{
+    if (n == 0) return 1.;
+    if (n == 1) return x;
+    if (n == 2) return x * x;
+    const double p  = x * x;
+    const double ep = fma(x, x, -p);
+    const double c  = p * x;
+    const double ec = fma(p, x, -c);
+    return c + (ec + ep * x);
+}
+__device__ __forceinline__ double factorial_count(int n) { return n == 2 ? 2. : n == 3 ? 6. : n == 4 ? 24. : 1.; }
+
+// ---------------------------------------------------------------------------------------------
+// Kernel configuration per network
+// ---------------------------------------------------------------------------------------------
+template<int NET>
+struct Cfg;
+template<>
+struct Cfg<14>
+{
+    static constexpr int D = 14, N = 15, NR = 29, NE = 16, NT = 64, TG = 8, S = 2;
+};
+template<>
+struct Cfg<86>
+{
+    static constexpr int D = 86, N = 87, NR = 309, NE = 157, NT = 256, TG = 16, S = 6;
+};
+template<>
+struct Cfg<87>
+{
+    static constexpr int D = 87, N = 88, NR = 311, NE = 157, NT = 256, TG = 16, S = 6;
+};
+
+template<int NET>
+struct Lds
+{
+    using C = Cfg<NET>;
+    static constexpr int D = C::D, N = C::N, NR = C::NR, NE = C::NE;
+    static constexpr int NU = N * (N + 1) / 2; // packed normalised rows
+    // phase-1/2 arrays (rates and assembly), aliased with the packed rows of the LU
+    static constexpr int oBE = 0, oMu = oBE + D, oEff = oMu + D, oDeff = oEff + NE, oL = oDeff + NE, oDl = oL + NE,
+                         oRate = oDl + NE, oDrate = oRate + NR, oVal = oDrate + NR, oDval = oVal + NR,
+                         oTr = oDval + NR, oJok = oTr + NR * MAXQ, oDYdot = oJok + NR, oDYdT = oDYdot + D,
+                         oCorr = oDYdT + D, oEv = oCorr + D, oEos = oEv + 12, phaseEnd = oEos + 4;
+    static constexpr int unionSize = phaseEnd > NU ? phaseEnd : NU;
+    static constexpr int oY = unionSize, oYb = oY + D, oX = oYb + D, oW = oX + N + 1, total = oW + 2 * C::TG * C::S;
+};
+
+// ---------------------------------------------------------------------------------------------
+// The FP64 kernel: one workgroup integrates one particle over one hydro step.
+// ---------------------------------------------------------------------------------------------
+template<int NET, int EOSK>
+__global__ void __launch_bounds__(Cfg<NET>::NT) nnet_fp64_kernel(NetDev nd, EosDev ed, BatchDev bd, StepParams sp)
+{
+    using C  = Cfg<NET>;
+    using L  = Lds<NET>;
+    constexpr int D = C::D, N = C::N, NT = C::NT, TG = C::TG, S = C::S;
+    constexpr int Dr = (NET == 14) ? 14 : 86; // species of the rate functor's BE/screening loops
+
+    __shared__ double sm[L::total];
+    double* sBE   = sm + L::oBE;
+    double* sMu   = sm + L::oMu;
+    double* sEff  = sm + L::oEff;
+    double* sDeff = sm + L::oDeff;
+    double* sL    = sm + L::oL;
+    double* sDl   = sm + L::oDl;
+    double* sRate = sm + L::oRate;
+    double* sDrt  = sm + L::oDrate;
+    double* sVal  = sm + L::oVal;
+    double* sDval = sm + L::oDval;
+    double* sTr   = sm + L::oTr;
+    double* sJok  = sm + L::oJok;
+    double* sDYd  = sm + L::oDYdot;
+    double* sDYT  = sm + L::oDYdT;
+    double* sCorr = sm + L::oCorr;
+    double* sEv   = sm + L::oEv;
+    double* sEos  = sm + L::oEos;
+    double* sU    = sm;
+    double* sY    = sm + L::oY;
+    double* sYb   = sm + L::oYb;
+    double* sX    = sm + L::oX;
+    double* sW    = sm + L::oW;
+
+    const int    tid = threadIdx.x;
+    const int    tr = tid / TG, tc = tid % TG;
+    const size_t p   = blockIdx.x;
+    if (p >= bd.n) return;
+
+    const double rho = bd.rho[p];
+    double       T   = bd.temp[p];
+    double       dt  = bd.dt[p];
+
+    // parallel_nuclear_net_impl.cpp:83
+    if (!(rho > k::minRho && T > k::minTemp))
+    {
+        if (tid == 0)
+        {
+            if (bd.status) bd.status[p] = NNET_STATUS_SKIPPED;
+            if (bd.substeps) bd.substeps[p] = 0;
+            if (bd.newton) bd.newton[p] = 0;
+        }
+        return;
+    }
+    // parallel_nuclear_net_impl.cpp:89-92 (the CPU path integrates with drho/dt = 0)
+    if (tid == 0 && sp.use_drhodt)
+        if (bd.rho_m1[p] != 0) bd.rho_m1[p] = (rho - bd.rho_m1[p]) / sp.previous_dt;
+    const double drho_dt = 0;
+
+    for (int s = tid; s < D; s += NT)
+        sY[s] = bd.Y[s][p];
+
+    const double dt_tot  = sp.hydro_dt;
+    double       elapsed = 0;
+    double       T_buf   = __builtin_nan("");
+    double       att_dt = 0, att_Tb = 0;
+    long         total_iter = 0;
+    uint32_t     n_sub = 0, n_newton = 0;
+    uint8_t      status = NNET_STATUS_OK;
+
+    for (int i = 1;; ++i)
+    {
+        if (i == 1)
+        {
+            att_dt = dt;
+            att_Tb = T_buf;
+        }
+        // prepareSystemSubstep (nuclear_net.hpp:858-879)
+        const double rho_t   = rho - drho_dt * (dt_tot - elapsed);
+        double       used_dt = dt;
+        if (dt_tot - elapsed < dt) used_dt = dt_tot - elapsed;
+        if (used_dt == 0)
+        {
+            status = NNET_STATUS_ZERO_DT;
+            break;
+        }
+        // prepareSystemNR (nuclear_net.hpp:699-712)
+        const double T_loc   = (i <= 1) ? T : T_buf;
+        const double T_theta = (1 - k::theta) * T + k::theta * T_loc;
+        for (int s = tid; s < D; s += NT)
+        {
+            const double yb = (i <= 1) ? sY[s] : sYb[s];
+            sYb[s]          = (1 - k::theta) * sY[s] + k::theta * yb;
+        }
+        __syncthreads();
+
+        // ---- EOS ----
+        double cv = 0., dpdT = 0.;
+        if constexpr (EOSK == NNET_EOS_IDEAL)
+        {
+            const double dmy = 8.317e7 / ed.mu;
+            cv               = 1.5 * dmy;
+            dpdT             = rho_t * dmy;
+        }
+        else
+        {
+            if (tid == 0)
+            {
+                double abar = 0., zbar = 0;
+                for (int s = 0; s < ed.zdim; ++s)
+                    abar += sYb[s];
+                for (int s = 0; s < ed.zdim; ++s)
+                    zbar += sYb[s] * ed.Z[s];
+                double c0, d0;
+                helmholtz_cv_dpdt(ed.grid, ed.cells, abar, zbar, T_theta, rho_t, c0, d0);
+                sEos[0] = c0;
+                sEos[1] = d0;
+            }
+        }
+
+        // ---- rates, phase 1 ----
+        {
+            const double Tg   = T_theta;
+            const int    SB   = nd.nfit;      // special groups
+            const int    SS   = nd.nfit + 6;  // species
+            if (tid < nd.nfit)
+            {
+                const int     f  = tid;
+                const double* fc = nd.fit + 8 * f;
+                const double  t9 = Tg / 1.e9, t913 = pow(t9, 1. / 3.), t923 = t913 * t913, t953 = pow(t9, 5. / 3.);
+                const double  t9i = 1.e0 / t9, t9i2 = t9i * t9i, t9i13 = 1.e0 / t913, t9i23 = t9i13 * t9i13,
+                             t9i43 = t9i23 * t9i23, lt9 = log(t9);
+                const double coef = fc[1] * t9i + fc[2] * t9i13 + fc[3] * t913 + fc[4] * t9 + fc[5] * t953 + fc[6] * lt9;
+                const double dcoef =
+                    (-fc[1] * t9i2 + (-fc[2] * t9i43 + fc[3] * t9i23 + 5 * fc[5] * t923) * (1. / 3.) + fc[4] +
+                     fc[6] * t9i) *
+                    1e-9;
+                const int    e   = nd.fit_e0 + f;
+                const double eff = exp(fc[0] + coef);
+                sEff[e]          = eff;
+                sDeff[e]         = eff * dcoef;
+                // reverse rate (net14.cpp:150-190, net86.cpp:524-554)
+                const double rt9  = nd.rev_div ? Tg / 1.e9 : Tg * 1e-09;
+                const double rt9i = 1. / rt9;
+                const double val1 = 11.6045 * rt9i;
+                const double val3 = val1 * rt9i * 1e-9;
+                const int    kb   = temperature_bin(Tg);
+                const double part = nd.choose[nd.ca[f] * 24 + kb] / nd.choose[nd.cb[f] * 24 + kb];
+                double       l, dl;
+                if (nd.photo[f])
+                {
+                    const double val2 = 1.5 * log(rt9);
+                    const double val4 = 1.5e-9 * rt9i;
+                    l                 = part * exp(fc[7] + coef - val1 * nd.q[f] + val2);
+                    dl                = l * (dcoef + val3 * nd.q[f] + val4);
+                }
+                else
+                {
+                    l  = part * exp(fc[7] + coef - val1 * nd.q[f]);
+                    dl = l * (dcoef + val3 * nd.q[f]);
+                }
+                sL[e]  = l;
+                sDl[e] = dl;
+            }
+            else if (tid >= SB && tid < SB + 6)
+            {
+                const int g = tid - SB;
+                double    r = 0, rl = 0, dr = 0, drl = 0;
+                if (g == 0)
+                {
+                    special_triple_alpha(Tg, r, rl, dr, drl);
+                    sEff[nd.sp.r3a_e] = r, sDeff[nd.sp.r3a_e] = dr, sL[nd.sp.r3a_l] = rl, sDl[nd.sp.r3a_l] = drl;
+                }
+                else if (g == 1)
+                {
+                    special_c12c12(Tg, r, dr);
+                    sEff[nd.sp.r24] = r, sDeff[nd.sp.r24] = dr;
+                }
+                else if (g == 2)
+                {
+                    special_c12o16(Tg, r, dr);
+                    sEff[nd.sp.r1216] = r, sDeff[nd.sp.r1216] = dr;
+                }
+                else if (g == 3)
+                {
+                    special_o16o16(Tg, r, dr);
+                    sEff[nd.sp.r32] = r, sDeff[nd.sp.r32] = dr;
+                }
+                else if (g == 4)
+                {
+                    special_c12ag(Tg, r, rl, dr, drl);
+                    sEff[nd.sp.rcag_e] = r, sDeff[nd.sp.rcag_e] = dr, sL[nd.sp.rcag_l] = rl, sDl[nd.sp.rcag_l] = drl;
+                }
+                else
+                {
+                    special_o16ag(Tg, r, rl, dr, drl);
+                    sEff[nd.sp.roag_e] = r, sDeff[nd.sp.roag_e] = dr, sL[nd.sp.roag_l] = rl, sDl[nd.sp.roag_l] = drl;
+                }
+            }
+            else if (tid >= SS && tid < SS + Dr)
+            {
+                // corrected binding energy (net14.cpp:49-75) and screening mu/kT (:577-600)
+                const int    s          = tid - SS;
+                const double kbt        = phys::Kb * Tg;
+                const double nakbt      = phys::Na * kbt;
+                const double correction = -1.5 * nakbt;
+                const double ne         = rho_t * phys::Na / 2.;
+                {
+                    const double ae  = pow((3. / 4.) / (phys::pi * ne), 1. / 3.);
+                    const double gam = phys::e2 / (kbt * ae);
+                    const double gmm = gam * nd.Z53[s];
+                    const double fn  = gmm > 1 ? coulomb_strong(gmm) : coulomb_weak(gmm);
+                    double       be  = nd.BE0[s] + correction;
+                    be -= nakbt * fn;
+                    sBE[s] = be;
+                }
+                {
+                    const double kbt2 = Tg * phys::Kb;
+                    const double ae   = pow(3. / (4. * phys::pi * ne), 1. / 3.);
+                    const double gam  = phys::e2 / (kbt2 * ae);
+                    const double a1 = -.898004, b1 = .96786, c1 = .220703, d1 = -.86097, e1 = 2.520058332;
+                    const double gp  = gam * nd.Z53[s];
+                    const double sq  = sqrt(gp);
+                    const double sq2 = sqrt(sq);
+                    double       mu;
+                    if (gp <= 1)
+                        mu = -(1. / 1.7320508075688772) * gp * sq + pow(gp, 1.9885) * .29561 / 1.9885;
+                    else
+                        mu = a1 * gp + 4. * (b1 * sq2 - c1 / sq2) + d1 * log(gp) - e1;
+                    if (NET != 14 && s == 1) mu = 0; // neutron (net86.cpp:603)
+                    sMu[s] = mu;
+                }
+            }
+            else if (NET == 87 && tid == SS + Dr)
+            {
+                electron_interpolate(nd.elogT, nd.elogR, nd.etab, Tg, sYb[86] * rho_t, sEv);
+            }
+            if (NET == 86 || NET == 87)
+                if (tid == NT - 1) sEff[3] = sDeff[3] = sL[3] = sDl[3] = 0; // unused slots
+            if (NET != 14)
+                if (tid == NT - 2) sL[0] = sL[1] = sL[2] = sDl[0] = sDl[1] = sDl[2] = 0;
+        }
+        __syncthreads();
+
+        // ---- rates, phase 2: swap, screening, reaction-list order ----
+        {
+            const double Tg = T_theta;
+            if (tid < nd.nE)
+            {
+                const int i  = tid;
+                double    e  = sEff[i], de = sDeff[i], l = sL[i], dl = sDl[i];
+                if (nd.swap[i])
+                {
+                    double t = e;
+                    e        = l;
+                    l        = t;
+                    t        = de;
+                    de       = dl;
+                    dl       = t;
+                }
+                const int* sc = nd.scr + 4 * i;
+                double     dm = sMu[sc[0]] + sMu[sc[1]] - sMu[sc[2]];
+                if (sc[3] >= 0) dm += sMu[sc[3]];
+                const double EF = exp(dm);
+                e               = e * EF;
+                de              = de * EF - 2. * e * dm / Tg;
+                if (nd.scr_l[i])
+                {
+                    l  = l * EF;
+                    dl = dl * EF - 2. * l * dm / Tg;
+                }
+                if (nd.push_e[i] >= 0) sRate[nd.push_e[i]] = e, sDrt[nd.push_e[i]] = de;
+                if (nd.push_l[i] >= 0) sRate[nd.push_l[i]] = l, sDrt[nd.push_l[i]] = dl;
+            }
+            if (NET == 87 && tid == NT - 1)
+            {
+                // net87.hpp:153-174
+                const double kbt        = phys::Kb * Tg;
+                const double nakbt      = phys::Na * kbt;
+                const double correction = -1.5 * nakbt;
+                sBE[86]                 = nd.BEe + correction;
+                sBE[0] += -sEv[3];
+                sBE[1] += -sEv[9];
+                const double rhoElec = sYb[86] * rho_t;
+                sRate[309]           = rhoElec == 0 ? 0 : sEv[0] / rhoElec;
+                sDrt[309]            = rhoElec == 0 ? 0 : sEv[1] / rhoElec;
+                sRate[310]           = rhoElec == 0 ? 0 : sEv[6] / rhoElec;
+                sDrt[310]            = rhoElec == 0 ? 0 : sEv[7] / rhoElec;
+            }
+        }
+        __syncthreads();
+
+        // ---- per-reaction mass-action factors (nuclear_net.hpp:355-371, 404-426) ----
+        for (int r = tid; r < nd.nR; r += NT)
+        {
+            double    v = sRate[r], dv = sDrt[r], jr = sRate[r];
+            const int q0 = nd.rb[r], q1 = nd.pb[r];
+            for (int q = q0; q < q1; ++q)
+            {
+                const int    c = nd.nums[q];
+                const double x = sYb[nd.ids[q]] * rho_t;
+                if (c != 1)
+                {
+                    const double fct = factorial_count(c);
+                    v                = v / fct;
+                    dv               = dv / fct;
+                    jr               = jr / fct;
+                }
+                v  = v * pow_count(x, c);
+                dv = dv * pow_count(x, c);
+                jr = jr * pow_count(x, c - 1);
+            }
+            sVal[r]  = v / rho_t;
+            sDval[r] = dv / rho_t;
+            const bool ok = jr > k::epsilonSystem;
+            sJok[r]       = ok ? 1. : 0.;
+            if (ok)
+                for (int qa = q0; qa < q1; ++qa)
+                {
+                    const int b  = nd.ids[qa];
+                    double    tr_ = jr;
+                    for (int o = q0; o < q1; ++o)
+                        if (nd.ids[o] != b) tr_ *= sYb[nd.ids[o]] * rho_t;
+                    sTr[r * MAXQ + (qa - q0)] = tr_;
+                }
+        }
+        __syncthreads();
+
+        // Jacobian block entry J[a][b] from its ordered contribution list.
+        auto jac = [&](int a, int b) -> double
+        {
+            double    J  = 0.;
+            const int e0 = nd.jp_ptr[a * D + b], e1 = nd.jp_ptr[a * D + b + 1];
+            for (int e = e0; e < e1; ++e)
+            {
+                const int r = nd.j_r[e];
+                if (sJok[r] == 0.) continue;
+                const int    c = nd.j_c[e];
+                const double t = sTr[r * MAXQ + nd.j_q[e]];
+                if (c < 0) J -= t * (double)(-c);
+                else
+                    J += t * (double)c;
+            }
+            return J;
+        };
+
+        // ---- species sums (derivativesFromReactions) and RHS corrections ----
+        if (tid < D)
+        {
+            const int s  = tid;
+            double    a  = 0., b = 0.;
+            const int e0 = nd.sp_ptr[s], e1 = nd.sp_ptr[s + 1];
+            for (int e = e0; e < e1; ++e)
+            {
+                const int r = nd.sp_r[e], c = nd.sp_c[e];
+                if (c < 0)
+                {
+                    a -= sVal[r] * (double)(-c);
+                    b -= sDval[r] * (double)(-c);
+                }
+                else
+                {
+                    a += sVal[r] * (double)c;
+                    b += sDval[r] * (double)c;
+                }
+            }
+            sDYd[s] = a;
+            sDYT[s] = b;
+        }
+        {
+            const int row = (NT >= 2 * D) ? tid - (NT - D) : tid;
+            if (row >= 0 && row < D)
+            {
+                double    corr = 0;
+                const int e0 = nd.row_ptr[row], e1 = nd.row_ptr[row + 1];
+                for (int e = e0; e < e1; ++e)
+                {
+                    const int b = nd.row_b[e];
+                    corr += jac(row, b) * (sYb[b] - sY[b]);
+                }
+                sCorr[row] = corr;
+            }
+        }
+        __syncthreads();
+
+        // ---- Newton matrix into registers (nuclear_net.hpp:551-608) ----
+        double cvv = cv, dpdTv = dpdT;
+        if constexpr (EOSK != NNET_EOS_IDEAL)
+        {
+            cvv   = sEos[0];
+            dpdTv = sEos[1];
+        }
+        const double mthdt   = -k::theta * used_dt;
+        const double thdt    = k::theta * used_dt;
+        const double drho    = drho_dt * used_dt;
+        const double value_1 = dpdTv * drho / (rho_t * rho_t);
+        double       M[S][S];
+#pragma unroll
+        for (int a = 0; a < S; ++a)
+#pragma unroll
+            for (int b = 0; b < S; ++b)
+            {
+                const int ri = tr + TG * a, ci = tc + TG * b;
+                double    v  = 0.;
+                if (ri < N && ci <= N)
+                {
+                    if (ri == 0)
+                    {
+                        if (ci == 0) v = 1 - k::theta * value_1 / cvv;
+                        else if (ci < N)
+                            v = -sBE[ci - 1] / cvv;
+                        else
+                            v = T * value_1 / cvv;
+                    }
+                    else
+                    {
+                        const int s = ri - 1;
+                        if (ci == 0) v = sDYT[s] * mthdt;
+                        else if (ci < N)
+                        {
+                            const double J = jac(s, ci - 1);
+                            v              = (ci == ri) ? 1. - thdt * J : mthdt * J;
+                        }
+                        else
+                        {
+                            double rhs = sDYd[s] * used_dt;
+                            rhs += mthdt * sDYT[s] * (T_theta - T);
+                            rhs += mthdt * sCorr[s];
+                            v = rhs;
+                        }
+                    }
+                }
+                M[a][b] = v;
+            }
+        __syncthreads(); // the packed rows below alias the assembly arrays
+
+        // ---- LU: right-looking, unpivoted, reference update order (eigen.hpp:182-206) ----
+#pragma unroll
+        for (int ka = 0; ka < S; ++ka)
+        {
+            for (int kr = 0; kr < TG; ++kr)
+            {
+                const int kk = ka * TG + kr;
+                if (kk >= N) break;
+                double*   Uk = sU + (kk * N - kk * (kk - 1) / 2); // packed row kk: columns kk+1 .. N
+                // phase A: normalise row kk (owners: tr == kr) and publish column kk
+                const double diag = __shfl(M[ka][ka], kr, TG);
+                if (tr == kr)
+                {
+#pragma unroll
+                    for (int b = ka; b < S; ++b)
+                    {
+                        const int ci = tc + TG * b;
+                        if (ci > kk && ci <= N) Uk[ci - kk - 1] = M[ka][b] / diag;
+                    }
+                }
+                if (tc == kr)
+                {
+#pragma unroll
+                    for (int a = ka; a < S; ++a)
+                    {
+                        const int ri = tr + TG * a;
+                        if (ri > kk && ri < N) sW[(kk & 1) * TG * S + ri] = M[a][ka];
+                    }
+                }
+                __syncthreads();
+                // phase B: rank-1 update of the trailing rows
+                double u[S];
+#pragma unroll
+                for (int b = ka; b < S; ++b)
+                {
+                    const int ci = tc + TG * b;
+                    u[b]         = (ci > kk && ci <= N) ? Uk[ci - kk - 1] : 0.;
+                }
+#pragma unroll
+                for (int a = ka; a < S; ++a)
+                {
+                    const int ri = tr + TG * a;
+                    if (ri > kk && ri < N)
+                    {
+                        const double w = sW[(kk & 1) * TG * S + ri];
+                        if (fabs(w) > k::epsilonSystem)
+                        {
+#pragma unroll
+                            for (int b = ka; b < S; ++b)
+                            {
+                                const int ci = tc + TG * b;
+                                if (ci > kk && ci <= N) M[a][b] = M[a][b] - w * u[b];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- back substitution on the packed unit-upper rows (first wavefront) ----
+        if (tid < 64)
+        {
+            constexpr int RS = (N + 63) / 64;
+            double        acc[RS];
+#pragma unroll
+            for (int s = 0; s < RS; ++s)
+            {
+                const int ri = tid + 64 * s;
+                acc[s]       = (ri < N) ? sU[ri * N - ri * (ri - 1) / 2 + (N - ri - 1)] : 0.;
+            }
+            for (int j = N - 1; j >= 0; --j)
+            {
+                const int    s  = j / 64;
+                double       xj = 0.;
+#pragma unroll
+                for (int ss = 0; ss < RS; ++ss)
+                    if (ss == s) xj = __shfl(acc[ss], j % 64, 64);
+                if (tid == 0) sX[j] = xj;
+#pragma unroll
+                for (int ss = 0; ss < RS; ++ss)
+                {
+                    const int ri = tid + 64 * ss;
+                    if (ri < j) acc[ss] = acc[ss] - sU[ri * N - ri * (ri - 1) / 2 + (j - ri - 1)] * xj;
+                }
+            }
+        }
+        __syncthreads();
+        ++n_newton;
+        ++total_iter;
+
+        // ---- finalizeSystemSubstep / finalizeSystemNR (nuclear_net.hpp:724-782, 887-922) ----
+        double u = dt;
+        if (dt_tot - elapsed < dt) u = dt_tot - elapsed;
+        double       timestep = 0;
+        bool         conv     = false;
+        const double last_T   = T_buf;
+        const double T_new    = T + sX[0];
+        int          nanflag  = 0;
+        double       yb_new   = 0.;
+        if (tid < D)
+        {
+            yb_new  = sY[tid] + sX[tid + 1];
+            nanflag = isnan(yb_new) ? 1 : 0;
+        }
+        const bool has_nan = __syncthreads_or(nanflag) || isnan(T_new);
+        if (has_nan || T_new < 0)
+        {
+            u *= k::nanDtStep;
+            i     = 0;
+            T_buf = T_new;
+            if (tid < D) sYb[tid] = yb_new;
+        }
+        else
+        {
+            const double dT_T = fabs((T_new - T) / T_new);
+            if (dT_T > k::NR_dTOverTempTarget * k::NR_dTOverTempTol)
+            {
+                u *= k::NR_dTOverTempTol / dT_T;
+                i     = 0;
+                T_buf = T;
+                if (tid < D) sYb[tid] = sY[tid];
+            }
+            else
+            {
+                if (tid < D) sYb[tid] = (yb_new <= k::epsilonVector) ? 0. : yb_new;
+                T_buf                  = T_new;
+                const double correction = fabs((T_buf - last_T) / T_buf);
+                if (correction < k::NR_itTol || i >= k::NR_maxIt)
+                {
+                    timestep = u;
+                    u        = (dT_T == 0 ? k::maxDtStep : k::NR_dTOverTempTarget / dT_T) * timestep;
+                    u        = dmin(u, timestep * (double)k::maxDtStep);
+                    u        = dmax(u, timestep * (double)k::minDtStep);
+                    u        = dmin(u, (double)k::NR_maxDt);
+                    conv     = true;
+                }
+            }
+        }
+        if (dt_tot - elapsed < dt)
+        {
+            if (u < dt_tot - elapsed) dt = u;
+        }
+        else
+            dt = u;
+        __syncthreads();
+        if (conv)
+        {
+            if (tid < D) sY[tid] = sYb[tid];
+            T = T_buf;
+            i = 0;
+            elapsed += timestep;
+            ++n_sub;
+            if ((dt_tot - elapsed) / dt_tot < k::substep_dtTol)
+            {
+                __syncthreads();
+                break;
+            }
+        }
+        else if (i == 0)
+        {
+            const bool same_dt = __double_as_longlong(dt) == __double_as_longlong(att_dt);
+            const bool same_T  = __double_as_longlong(T_buf) == __double_as_longlong(att_Tb);
+            if (same_dt && same_T)
+            {
+                status = NNET_STATUS_STALLED;
+                break;
+            }
+        }
+        if (sp.max_iter > 0 && total_iter >= sp.max_iter)
+        {
+            status = NNET_STATUS_MAX_ITER;
+            break;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int s = tid; s < D; s += NT)
+        bd.Y[s][p] = sY[s];
+    if (tid == 0)
+    {
+        bd.temp[p] = T;
+        bd.dt[p]   = dt;
+        if (bd.status) bd.status[p] = status;
+        if (bd.substeps) bd.substeps[p] = n_sub;
+        if (bd.newton) bd.newton[p] = n_newton;
+    }
+}
+
+} // namespace nnet_amd
+
+// =============================================================================================
+// Host side
+// =============================================================================================
+using namespace nnet_amd;
+
+struct nnet_ctx
+{
+    int         device = 0;
+    std::string err;
+    long        max_iter = 200000;
+    // Helmholtz
+    HelmGrid* d_grid  = nullptr;
+    double*   d_cells = nullptr;
+    bool      helm_ok = false;
+    // per-call scratch (grow only)
+    std::vector<void*> owned;
+};
+
+struct nnet_network
+{
+    nnet_ctx*        ctx = nullptr;
+    int              kind = 0, d = 0, nR = 0;
+    std::vector<int> rb, pb, ids, nums;
+    NetDev           dev{};
+    void*            blob = nullptr;
+};
+
+static int fail(nnet_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                                             \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t e_ = (expr);                                                                                        \
+        if (e_ != hipSuccess) return fail(ctx, NNET_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_));          \
+    } while (0)
+
+// The library's reaction lists (net14.hpp:69-134, net86.hpp:95-192, net87.hpp:88-98).
+struct HostList
+{
+    std::vector<int> rb{0}, pb, ids, nums;
+    void push(std::vector<std::pair<int, int>> r, std::vector<std::pair<int, int>> pr)
+    {
+        for (auto& x : r)
+            ids.push_back(x.first), nums.push_back(x.second);
+        for (auto& x : pr)
+            ids.push_back(x.first), nums.push_back(x.second);
+        pb.push_back(rb.back() + (int)r.size());
+        rb.push_back(pb.back() + (int)pr.size());
+    }
+};
+
+static HostList builtin_list(int kind)
+{
+    HostList L;
+    if (kind == 14)
+    {
+        L.push({{0, 3}}, {{1, 1}});
+        L.push({{0, 1}, {1, 1}}, {{2, 1}});
+        L.push({{0, 1}, {2, 1}}, {{3, 1}});
+        for (int s = 3; s < 13; ++s)
+            L.push({{0, 1}, {s, 1}}, {{s + 1, 1}});
+        L.push({{1, 2}}, {{3, 1}, {0, 1}});
+        L.push({{1, 1}, {2, 1}}, {{4, 1}, {0, 1}});
+        L.push({{2, 2}}, {{5, 1}, {0, 1}});
+        L.push({{1, 1}}, {{0, 3}});
+        L.push({{2, 1}}, {{0, 1}, {1, 1}});
+        L.push({{3, 1}}, {{0, 1}, {2, 1}});
+        for (int s = 4; s < 14; ++s)
+            L.push({{s, 1}}, {{0, 1}, {s - 1, 1}});
+        return L;
+    }
+    const int* mr = NNET86_MAIN_REACTANT;
+    const int* sr = NNET86_SECONDARY_REACTANT;
+    const int* mp = NNET86_MAIN_PRODUCT;
+    auto       other = [](int dZ, int dA) -> int
+    {
+        if (dZ == 0 && dA == 0) return -1;
+        if (dA == 1 && dZ == 0) return 1; // neutron
+        if (dA == 1 && dZ == 1) return 0; // proton
+        if (dA == 4 && dZ == 2) return 2; // alpha
+        return -2;
+    };
+    L.push({{mr[0], 2}}, {{mp[0], 1}, {2, 1}});
+    L.push({{mr[1], 1}, {sr[1], 1}}, {{mp[1], 1}, {2, 1}});
+    L.push({{mr[2], 2}}, {{mp[2], 1}, {2, 1}});
+    L.push({{mr[4], 3}}, {{mp[4], 1}});
+    for (int i = 5; i < 157; ++i)
+    {
+        const int o = other((int)(NNET86_Z[mr[i]] + NNET86_Z[sr[i]] - NNET86_Z[mp[i]]),
+                            (int)(NNET86_A[mr[i]] + NNET86_A[sr[i]] - NNET86_A[mp[i]]));
+        if (o == -2) return HostList{};
+        if (o < 0) L.push({{mr[i], 1}, {sr[i], 1}}, {{mp[i], 1}});
+        else
+            L.push({{mr[i], 1}, {sr[i], 1}}, {{mp[i], 1}, {o, 1}});
+    }
+    L.push({{mp[4], 1}}, {{mr[4], 3}});
+    for (int i = 5; i < 157; ++i)
+    {
+        const int o = other((int)((NNET86_Z[mr[i]] + NNET86_Z[sr[i]]) - NNET86_Z[mp[i]]),
+                            (int)((NNET86_A[mr[i]] + NNET86_A[sr[i]]) - NNET86_A[mp[i]]));
+        if (o == -2) return HostList{};
+        if (o < 0) L.push({{mp[i], 1}}, {{mr[i], 1}, {sr[i], 1}});
+        else
+            L.push({{mp[i], 1}, {o, 1}}, {{mr[i], 1}, {sr[i], 1}});
+    }
+    if (kind == 87)
+    {
+        L.push({{0, 1}, {86, 1}}, {{1, 1}});
+        L.push({{1, 1}, {86, 1}}, {{0, 1}});
+    }
+    return L;
+}
+
+// Byte blob with aligned sub-arrays, copied to the device in one transfer.
+struct Blob
+{
+    std::vector<char> b;
+    template<class T>
+    size_t add(const T* p, size_t n)
+    {
+        size_t off = (b.size() + 63) & ~size_t(63);
+        b.resize(off + n * sizeof(T));
+        if (n) std::memcpy(b.data() + off, p, n * sizeof(T));
+        return off;
+    }
+    template<class T>
+    size_t add(const std::vector<T>& v)
+    {
+        return add(v.data(), v.size());
+    }
+};
+
+static int build_network(nnet_ctx* ctx, nnet_network* nw)
+{
+    const int kind = nw->kind, d = nw->d, nR = nw->nR;
+    const int nrate = kind == 14 ? 29 : kind == 86 ? 309 : 311;
+    if (nR != nrate) return fail(ctx, NNET_E_NETWORK, "reaction count does not match the rate parameterization");
+    for (int r = 0; r < nR; ++r)
+    {
+        if (nw->pb[r] - nw->rb[r] > MAXQ || nw->pb[r] < nw->rb[r] || nw->rb[r + 1] < nw->pb[r])
+            return fail(ctx, NNET_E_NETWORK, "malformed reaction list");
+        for (int q = nw->rb[r]; q < nw->rb[r + 1]; ++q)
+            if (nw->ids[q] < 0 || nw->ids[q] >= d || nw->nums[q] < 1 || nw->nums[q] > 3)
+                return fail(ctx, NNET_E_NETWORK, "reaction entry out of range");
+    }
+
+    // ---- rate-parameterization tables ----
+    const int          Dr = kind == 14 ? 14 : 86;
+    std::vector<double> BE0(Dr), Zr(Dr), Z53(Dr);
+    for (int s = 0; s < Dr; ++s)
+    {
+        BE0[s] = (kind == 14 ? NNET14_BE_MEV[s] : NNET86_BE_MEV[s]) * NNET_MEV_TO_ERG;
+        Zr[s]  = kind == 14 ? NNET14_Z[s] : NNET86_Z[s];
+        Z53[s] = std::pow(Zr[s], 5. / 3.);
+    }
+    const int           nfit = kind == 14 ? 10 : 150, nE = kind == 14 ? 16 : 157, e0 = kind == 14 ? 3 : 7;
+    std::vector<double> fit(nfit * 8), q(nfit), choose;
+    std::vector<int>    ca(nfit), cb(nfit), photo(nfit), scr(4 * nE), swp(nE, 0), scl(nE, 0), pe(nE, -1), pl(nE, -1);
+    if (kind == 14)
+    {
+        std::memcpy(fit.data(), NNET14_FIT, sizeof(NNET14_FIT));
+        std::memcpy(q.data(), NNET14_Q, sizeof(NNET14_Q));
+        choose.assign(&NNET14_CHOOSE[0][0], &NNET14_CHOOSE[0][0] + 11 * 24);
+        for (int f = 0; f < nfit; ++f)
+            ca[f] = f, cb[f] = f + 1, photo[f] = 1;
+        for (int i = 0; i < 14; ++i)
+            scr[4 * i] = i, scr[4 * i + 1] = 0, scr[4 * i + 2] = i + 1, scr[4 * i + 3] = -1;
+        scr[3] = 0;
+        int o13[4] = {1, 1, 3, -1}, o14[4] = {1, 2, 4, -1}, o15[4] = {2, 2, 5, -1};
+        std::memcpy(&scr[4 * 13], o13, sizeof(o13));
+        std::memcpy(&scr[4 * 14], o14, sizeof(o14));
+        std::memcpy(&scr[4 * 15], o15, sizeof(o15));
+        for (int i = 0; i < 16; ++i)
+            pe[i] = i;
+        for (int i = 0; i < 13; ++i)
+            pl[i] = 16 + i;
+    }
+    else
+    {
+        std::memcpy(fit.data(), NNET86_FIT, sizeof(NNET86_FIT));
+        std::memcpy(q.data(), NNET86_Q, sizeof(NNET86_Q));
+        choose.assign(&NNET86_CHOOSE[0][0], &NNET86_CHOOSE[0][0] + 81 * 24);
+        for (int f = 0; f < nfit; ++f)
+        {
+            const int i = f + 7;
+            ca[f]       = NNET86_MAIN_REACTANT[i] - 5;
+            cb[f]       = NNET86_MAIN_PRODUCT[i] - 5;
+            photo[f]    = i < 137 ? 1 : 0;
+        }
+        for (int i = 0; i < 157; ++i)
+        {
+            scr[4 * i]     = NNET86_MAIN_REACTANT[i];
+            scr[4 * i + 1] = NNET86_SECONDARY_REACTANT[i];
+            scr[4 * i + 2] = NNET86_MAIN_PRODUCT[i];
+            scr[4 * i + 3] = i == 4 ? NNET86_MAIN_REACTANT[4] : -1;
+            swp[i]         = (i >= 137 && (i < 154 || i == 156)) ? 1 : 0;
+            scl[i]         = i >= 137 ? 1 : 0;
+        }
+        int idx = 0;
+        for (int i = 0; i < 157; ++i)
+            if (i != 3) pe[i] = idx++;
+        for (int i = 4; i < 157; ++i)
+            pl[i] = idx++;
+    }
+    SpecialIdx si = kind == 14 ? SpecialIdx{0, 0, 13, 14, 15, 1, 1, 2, 2} : SpecialIdx{4, 4, 0, 1, 2, 5, 5, 6, 6};
+
+    // ---- assembly maps ----
+    std::vector<int> sp_ptr(d + 1, 0), sp_r, sp_c;
+    {
+        std::vector<std::vector<std::pair<int, int>>> per(d);
+        for (int r = 0; r < nR; ++r)
+        {
+            for (int qq = nw->rb[r]; qq < nw->pb[r]; ++qq)
+                per[nw->ids[qq]].push_back({r, -nw->nums[qq]});
+            for (int qq = nw->pb[r]; qq < nw->rb[r + 1]; ++qq)
+                per[nw->ids[qq]].push_back({r, nw->nums[qq]});
+        }
+        for (int s = 0; s < d; ++s)
+        {
+            for (auto& x : per[s])
+                sp_r.push_back(x.first), sp_c.push_back(x.second);
+            sp_ptr[s + 1] = (int)sp_r.size();
+        }
+    }
+    std::vector<int> jp_ptr((size_t)d * d + 1, 0), j_r, j_q, j_c, row_ptr(d + 1, 0), row_b;
+    {
+        std::vector<std::vector<int>> er((size_t)d * d), eq((size_t)d * d), ec((size_t)d * d);
+        for (int r = 0; r < nR; ++r)
+            for (int qb = nw->rb[r]; qb < nw->pb[r]; ++qb)
+            {
+                const int b = nw->ids[qb];
+                for (int o = nw->rb[r]; o < nw->pb[r]; ++o)
+                {
+                    const size_t e = (size_t)nw->ids[o] * d + b;
+                    er[e].push_back(r), eq[e].push_back(qb - nw->rb[r]), ec[e].push_back(-nw->nums[o]);
+                }
+                for (int o = nw->pb[r]; o < nw->rb[r + 1]; ++o)
+                {
+                    const size_t e = (size_t)nw->ids[o] * d + b;
+                    er[e].push_back(r), eq[e].push_back(qb - nw->rb[r]), ec[e].push_back(nw->nums[o]);
+                }
+            }
+        for (size_t e = 0; e < (size_t)d * d; ++e)
+        {
+            for (size_t t = 0; t < er[e].size(); ++t)
+                j_r.push_back(er[e][t]), j_q.push_back(eq[e][t]), j_c.push_back(ec[e][t]);
+            jp_ptr[e + 1] = (int)j_r.size();
+        }
+        for (int a = 0; a < d; ++a)
+        {
+            for (int b = 0; b < d; ++b)
+                if (!er[(size_t)a * d + b].empty()) row_b.push_back(b);
+            row_ptr[a + 1] = (int)row_b.size();
+        }
+    }
+
+    // ---- upload ----
+    Blob bl;
+    size_t oBE = bl.add(BE0), oZ = bl.add(Zr), oZ53 = bl.add(Z53), oFit = bl.add(fit), oQ = bl.add(q),
+           oCh = bl.add(choose), oCa = bl.add(ca), oCb = bl.add(cb), oPh = bl.add(photo), oScr = bl.add(scr),
+           oSw = bl.add(swp), oScl = bl.add(scl), oPe = bl.add(pe), oPl = bl.add(pl),
+           oElT = bl.add(NNET_ELEC_LOGT, NNET_ELEC_NT), oElR = bl.add(NNET_ELEC_LOGRHO, NNET_ELEC_NR),
+           oElTab = bl.add(NNET_ELEC_RATE, (size_t)NNET_ELEC_NT * NNET_ELEC_NR * NNET_ELEC_NC), oRb = bl.add(nw->rb),
+           oPb = bl.add(nw->pb), oIds = bl.add(nw->ids), oNums = bl.add(nw->nums), oSpP = bl.add(sp_ptr),
+           oSpR = bl.add(sp_r), oSpC = bl.add(sp_c), oJpP = bl.add(jp_ptr), oJR = bl.add(j_r), oJQ = bl.add(j_q),
+           oJC = bl.add(j_c), oRowP = bl.add(row_ptr), oRowB = bl.add(row_b);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMalloc(&nw->blob, bl.b.size()));
+    HIP_TRY(ctx, hipMemcpy(nw->blob, bl.b.data(), bl.b.size(), hipMemcpyHostToDevice));
+    char*   B = (char*)nw->blob;
+    NetDev& n = nw->dev;
+    n.net = kind, n.d = d, n.nR = nR, n.nE = nE, n.nfit = nfit, n.fit_e0 = e0, n.rev_div = kind == 14 ? 1 : 0;
+    n.sp = si;
+    n.BE0 = (double*)(B + oBE), n.Zr = (double*)(B + oZ), n.Z53 = (double*)(B + oZ53), n.fit = (double*)(B + oFit);
+    n.q = (double*)(B + oQ), n.choose = (double*)(B + oCh), n.ca = (int*)(B + oCa), n.cb = (int*)(B + oCb);
+    n.photo = (int*)(B + oPh), n.scr = (int*)(B + oScr), n.swap = (int*)(B + oSw), n.scr_l = (int*)(B + oScl);
+    n.push_e = (int*)(B + oPe), n.push_l = (int*)(B + oPl), n.elogT = (double*)(B + oElT);
+    n.elogR = (double*)(B + oElR), n.etab = (double*)(B + oElTab), n.BEe = NNET87_ELECTRON_BE_MEV[0] * NNET_MEV_TO_ERG;
+    n.rb = (int*)(B + oRb), n.pb = (int*)(B + oPb), n.ids = (int*)(B + oIds), n.nums = (int*)(B + oNums);
+    n.sp_ptr = (int*)(B + oSpP), n.sp_r = (int*)(B + oSpR), n.sp_c = (int*)(B + oSpC);
+    n.jp_ptr = (int*)(B + oJpP), n.j_r = (int*)(B + oJR), n.j_q = (int*)(B + oJQ), n.j_c = (int*)(B + oJC);
+    n.row_ptr = (int*)(B + oRowP), n.row_b = (int*)(B + oRowB);
+    return NNET_OK;
+}
+
+extern "C"
+{
+    int nnet_ctx_create(nnet_ctx** out, int device, const char* helm_table_path)
+    {
+        if (!out) return NNET_E_ARG;
+        *out       = nullptr;
+        nnet_ctx* c = new nnet_ctx;
+        c->device  = device;
+        int ndev   = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        {
+            delete c;
+            return NNET_E_HIP;
+        }
+        if (hipSetDevice(device) != hipSuccess)
+        {
+            delete c;
+            return NNET_E_HIP;
+        }
+        *out = c;
+        if (helm_table_path) return nnet_ctx_load_helmholtz(c, helm_table_path);
+        return NNET_OK;
+    }
+
+    void nnet_ctx_destroy(nnet_ctx* c)
+    {
+        if (!c) return;
+        (void)hipSetDevice(c->device);
+        if (c->d_grid) (void)hipFree(c->d_grid);
+        if (c->d_cells) (void)hipFree(c->d_cells);
+        for (void* p : c->owned)
+            (void)hipFree(p);
+        delete c;
+    }
+
+    const char* nnet_last_error(const nnet_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+    int nnet_ctx_set_max_iterations(nnet_ctx* c, long m)
+    {
+        if (!c || m < 0) return NNET_E_ARG;
+        c->max_iter = m;
+        return NNET_OK;
+    }
+
+    int nnet_ctx_load_helmholtz(nnet_ctx* c, const char* path)
+    {
+        if (!c || !path) return NNET_E_ARG;
+        FILE* f = std::fopen(path, "rb");
+        if (!f) return fail(c, NNET_E_TABLE, std::string("cannot open Helmholtz table ") + path);
+        const size_t        cells = (size_t)HELM_IMAX * HELM_JMAX;
+        std::vector<double> arr((size_t)21 * cells);
+        const size_t        got = std::fread(arr.data(), sizeof(double), arr.size(), f);
+        std::fclose(f);
+        if (got != arr.size()) return fail(c, NNET_E_TABLE, "Helmholtz table too short");
+        // grid (helmholtz.cpp:101-169)
+        HelmGrid g{};
+        for (int i = 0; i < HELM_IMAX; ++i)
+            g.d[i] = std::pow((double)10., helmc::dlo + i * helmc::dstp);
+        for (int j = 0; j < HELM_JMAX; ++j)
+            g.t_[j] = std::pow((double)10., helmc::tlo + j * helmc::tstp);
+        for (int j = 0; j < HELM_JMAX - 1; ++j)
+        {
+            const double dth = g.t_[j + 1] - g.t_[j], dt2 = dth * dth;
+            g.dt_sav[j] = dth, g.dt2_sav[j] = dt2, g.dti_sav[j] = 1. / dth, g.dt2i_sav[j] = 1. / dt2;
+        }
+        for (int i = 0; i < HELM_IMAX - 1; ++i)
+        {
+            const double dd = g.d[i + 1] - g.d[i], dd2 = dd * dd;
+            g.dd_sav[i] = dd, g.dd2_sav[i] = dd2, g.ddi_sav[i] = 1. / dd;
+        }
+        // 4-corner stencil per cell, coefficient order of helmholtz.cpp:431-466
+        const int           ks[9] = {0, 2, 4, 1, 3, 5, 6, 7, 8}; // f, ft, ftt, fd, fdd, fdt, fddt, fdtt, fddtt
+        std::vector<double> st(cells * HELM_CELL, 0.);
+        for (int i = 0; i + 1 < HELM_IMAX; ++i)
+            for (int j = 0; j + 1 < HELM_JMAX; ++j)
+            {
+                double* o = &st[((size_t)i * HELM_JMAX + j) * HELM_CELL];
+                for (int qq = 0; qq < 9; ++qq)
+                {
+                    const double* a = &arr[(size_t)ks[qq] * cells];
+                    o[4 * qq + 0]   = a[(size_t)(i + 0) * HELM_JMAX + (j + 0)];
+                    o[4 * qq + 1]   = a[(size_t)(i + 1) * HELM_JMAX + (j + 0)];
+                    o[4 * qq + 2]   = a[(size_t)(i + 0) * HELM_JMAX + (j + 1)];
+                    o[4 * qq + 3]   = a[(size_t)(i + 1) * HELM_JMAX + (j + 1)];
+                }
+            }
+        HIP_TRY(c, hipSetDevice(c->device));
+        if (!c->d_grid) HIP_TRY(c, hipMalloc(&c->d_grid, sizeof(HelmGrid)));
+        if (!c->d_cells) HIP_TRY(c, hipMalloc(&c->d_cells, st.size() * sizeof(double)));
+        HIP_TRY(c, hipMemcpy(c->d_grid, &g, sizeof(HelmGrid), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->d_cells, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice));
+        c->helm_ok = true;
+        return NNET_OK;
+    }
+
+    int nnet_network_create(nnet_ctx* c, int kind, int n_species, int n_reactions, const int* reactant_begin,
+                            const int* product_begin, const int* species_id, const int* count, nnet_network** out)
+    {
+        if (!c || !out || !reactant_begin || !product_begin || !species_id || !count) return NNET_E_ARG;
+        *out = nullptr;
+        if (!(kind == 14 || kind == 86 || kind == 87)) return fail(c, NNET_E_ARG, "unknown rate parameterization");
+        const int dexp = kind == 14 ? 14 : kind == 86 ? 86 : 87;
+        if (n_species != dexp) return fail(c, NNET_E_NETWORK, "species count does not match the rate parameterization");
+        nnet_network* nw = new nnet_network;
+        nw->ctx = c, nw->kind = kind, nw->d = n_species, nw->nR = n_reactions;
+        nw->rb.assign(reactant_begin, reactant_begin + n_reactions + 1);
+        nw->pb.assign(product_begin, product_begin + n_reactions);
+        nw->ids.assign(species_id, species_id + nw->rb.back());
+        nw->nums.assign(count, count + nw->rb.back());
+        const int rc = build_network(c, nw);
+        if (rc != NNET_OK)
+        {
+            if (nw->blob) (void)hipFree(nw->blob);
+            delete nw;
+            return rc;
+        }
+        *out = nw;
+        return NNET_OK;
+    }
+
+    int nnet_network_builtin(nnet_ctx* c, int kind, nnet_network** out)
+    {
+        if (!c || !out) return NNET_E_ARG;
+        if (!(kind == 14 || kind == 86 || kind == 87)) return fail(c, NNET_E_ARG, "unknown rate parameterization");
+        HostList L = builtin_list(kind);
+        if (L.pb.empty()) return fail(c, NNET_E_NETWORK, "could not build the reaction list");
+        const int d = kind == 14 ? 14 : kind == 86 ? 86 : 87;
+        return nnet_network_create(c, kind, d, (int)L.pb.size(), L.rb.data(), L.pb.data(), L.ids.data(),
+                                   L.nums.data(), out);
+    }
+
+    void nnet_network_destroy(nnet_network* nw)
+    {
+        if (!nw) return;
+        if (nw->blob) (void)hipFree(nw->blob);
+        delete nw;
+    }
+
+    int nnet_network_reactions(const nnet_network* nw, int* n_reactions, int* n_entries, int* reactant_begin,
+                               int* product_begin, int* species_id, int* count)
+    {
+        if (!nw) return NNET_E_ARG;
+        if (n_reactions) *n_reactions = nw->nR;
+        if (n_entries) *n_entries = (int)nw->ids.size();
+        if (reactant_begin) std::memcpy(reactant_begin, nw->rb.data(), nw->rb.size() * sizeof(int));
+        if (product_begin) std::memcpy(product_begin, nw->pb.data(), nw->pb.size() * sizeof(int));
+        if (species_id) std::memcpy(species_id, nw->ids.data(), nw->ids.size() * sizeof(int));
+        if (count) std::memcpy(count, nw->nums.data(), nw->nums.size() * sizeof(int));
+        return NNET_OK;
+    }
+
+    int nnet_compute(nnet_ctx* c, const nnet_batch* batches, int nb, double hydro_dt, double previous_dt,
+                     int use_drhodt, void* stream)
+    {
+        if (!c || (nb > 0 && !batches) || nb < 0) return NNET_E_ARG;
+        if (!(hydro_dt > 0)) return fail(c, NNET_E_ARG, "hydro_dt must be positive");
+        HIP_TRY(c, hipSetDevice(c->device));
+        hipStream_t st = (hipStream_t)stream;
+        StepParams  sp{hydro_dt, previous_dt, use_drhodt, c->max_iter};
+        for (int bi = 0; bi < nb; ++bi)
+        {
+            const nnet_batch& b = batches[bi];
+            if (!b.network || b.network->ctx != c) return fail(c, NNET_E_ARG, "batch without a network of this context");
+            if (b.n == 0) continue;
+            if (!b.rho || !b.Y || !b.temp || !b.dt || (use_drhodt && !b.rho_m1))
+                return fail(c, NNET_E_ARG, "null particle array");
+            if (b.precision != NNET_PRECISION_FP64 && b.precision != NNET_PRECISION_MIXED)
+                return fail(c, NNET_E_ARG, "unknown precision");
+            const int d = b.network->d;
+            for (int s = 0; s < d; ++s)
+                if (!b.Y[s]) return fail(c, NNET_E_ARG, "null abundance array");
+            EosDev ed{};
+            ed.kind = b.eos;
+            ed.mu   = b.mu;
+            if (b.eos == NNET_EOS_HELMHOLTZ)
+            {
+                if (!c->helm_ok) return fail(c, NNET_E_STATE, "no Helmholtz table loaded");
+                if (!b.Z || b.z_dim <= 0 || b.z_dim > d) return fail(c, NNET_E_ARG, "Helmholtz Z/z_dim invalid");
+            }
+            else if (b.eos != NNET_EOS_IDEAL)
+                return fail(c, NNET_E_ARG, "unknown EOS");
+            if (b.eos == NNET_EOS_HELMHOLTZ)
+            {
+                for (int z = 0; z < b.z_dim; ++z)
+                    ed.Z[z] = b.Z[z];
+                ed.zdim  = b.z_dim;
+                ed.grid  = c->d_grid;
+                ed.cells = c->d_cells;
+            }
+            BatchDev bd{};
+            bd.n = b.n, bd.rho = b.rho, bd.rho_m1 = b.rho_m1, bd.temp = b.temp, bd.dt = b.dt;
+            bd.status = b.status, bd.substeps = b.substeps, bd.newton = b.newton;
+            for (int z = 0; z < d; ++z)
+                bd.Y[z] = b.Y[z];
+            const NetDev& nd = b.network->dev;
+            if (b.n > 0x7fffffffULL) return fail(c, NNET_E_ARG, "batch too large for one launch");
+            const dim3 grid((unsigned)b.n);
+            const int  kind = b.network->kind;
+            if (kind == 14)
+            {
+                if (b.eos == NNET_EOS_IDEAL)
+                    hipLaunchKernelGGL((nnet_fp64_kernel<14, NNET_EOS_IDEAL>), grid, dim3(Cfg<14>::NT), 0, st, nd, ed, bd, sp);
+                else
+                    hipLaunchKernelGGL((nnet_fp64_kernel<14, NNET_EOS_HELMHOLTZ>), grid, dim3(Cfg<14>::NT), 0, st, nd, ed, bd,
+                                       sp);
+            }
+            else if (kind == 86)
+            {
+                if (b.eos == NNET_EOS_IDEAL)
+                    hipLaunchKernelGGL((nnet_fp64_kernel<86, NNET_EOS_IDEAL>), grid, dim3(Cfg<86>::NT), 0, st, nd, ed, bd, sp);
+                else
+                    hipLaunchKernelGGL((nnet_fp64_kernel<86, NNET_EOS_HELMHOLTZ>), grid, dim3(Cfg<86>::NT), 0, st, nd, ed, bd,
+                                       sp);
+            }
+            else
+            {
+                if (b.eos == NNET_EOS_IDEAL)
+                    hipLaunchKernelGGL((nnet_fp64_kernel<87, NNET_EOS_IDEAL>), grid, dim3(Cfg<87>::NT), 0, st, nd, ed, bd, sp);
+                else
+                    hipLaunchKernelGGL((nnet_fp64_kernel<87, NNET_EOS_HELMHOLTZ>), grid, dim3(Cfg<87>::NT), 0, st, nd, ed, bd,
+                                       sp);
+            }
+            HIP_TRY(c, hipGetLastError());
+        }
+        return NNET_OK;
+    }
+
+    int nnet_compute_sync(nnet_ctx* c, const nnet_batch* batches, int nb, double hydro_dt, double previous_dt,
+                          int use_drhodt)
+    {
+        int rc = nnet_compute(c, batches, nb, hydro_dt, previous_dt, use_drhodt, nullptr);
+        if (rc != NNET_OK) return rc;
+        HIP_TRY(c, hipDeviceSynchronize());
+        return NNET_OK;
+    }
+}

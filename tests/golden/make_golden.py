@@ -1,0 +1,121 @@
+#!/usr/bin/env python3
+"""Mint golden vectors from the reference build (oracle/_ref/libnnet_ref.so, compiled from the
+SPH-EXA-nnet sources by oracle/Makefile.ref).  Run only where the reference tree exists:
+
+    make -C oracle -f Makefile.ref && python tests/golden/make_golden.py
+
+Writes small .npz fixtures next to this script: reaction lists, rates / dR/dT / corrected BE on a
+(T, rho) grid that crosses every branch point of the rate code, eigen::solve solutions, assembled
+Newton systems, and whole-particle substep trajectories (ideal gas; including controller
+replay stalls and the isothermal mu = 1e-20 case).  Helmholtz fixtures are added by
+make_golden_helm() once a table is available (the reference needs one at compile time).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "sph-exa-nnet_amd")]
+
+from oracle import Reference  # noqa: E402
+import workloads  # noqa: E402
+
+DIMS = {14: 14, 86: 86, 87: 87}
+
+
+def rate_points():
+    # temperatures straddling the branch points: 8e7 (3-alpha form), 5e8 (12C+16O on),
+    # the 24 partition-function bins, 1e10 (t9 clamp)
+    T = [9e7, 1.0e8, 1.49e8, 1.5e8, 4.99e8, 5e8, 5.01e8, 8e8, 1e9, 1.7e9, 2.5e9, 3.3e9, 4.5e9, 5e9, 6.5e9, 9.9e9,
+         1e10, 1.3e10, 3e10]
+    rho = [1e4, 3e5, 1e7, 2e8, 5e9, 1e10, 7e5, 4e6, 1e8, 3e9, 1e5, 5e7, 8e8, 2e9, 6e4, 1e6, 9e9, 4e8, 2e7]
+    return np.array(T), np.array(rho)
+
+
+def main():
+    ref = Reference()
+    rng = np.random.default_rng(424242)
+    out = {}
+    for net in (14, 86, 87):
+        d = DIMS[net]
+        rb, pb, ids, nums = ref.reaction_list(net)
+        T, rho = rate_points()
+        Ys = np.zeros((len(T), d))
+        if net == 87:
+            Ys[:, 86] = rng.uniform(0.0, 0.5, len(T))
+            Ys[3, 86] = 0.0  # rhoElec == 0 branch of net87.hpp:170-174
+        BE, R, DR = [], [], []
+        for k in range(len(T)):
+            be, r, dr = ref.rates(net, Ys[k], T[k], rho[k])
+            BE.append(be), R.append(r), DR.append(dr)
+        np.savez_compressed(os.path.join(HERE, f"rates_net{net}.npz"), T=T, rho=rho, Y=Ys, BE=np.array(BE),
+                            rates=np.array(R), drates=np.array(DR), rb=rb, pb=pb, ids=ids, nums=nums)
+
+        # assembled Newton systems (ideal gas, mu = 10)
+        Ms, Rs, ins = [], [], []
+        o14 = workloads.net14_species_order()
+        for k in range(6):
+            Y = np.zeros(d)
+            if net == 14:
+                Y[:4] = rng.dirichlet(np.full(4, 0.5)) / workloads.A14[:4]
+            else:
+                Y[o14] = rng.dirichlet(np.full(14, 0.3)) / workloads.A14
+                if net == 87:
+                    Y[86] = rng.uniform(0.45, 0.5)
+            Yg = np.abs(Y * (1 + 0.05 * rng.standard_normal(d)))
+            Tv = 10 ** rng.uniform(8.5, 9.9)
+            Tg = Tv * (1 + 0.01 * rng.standard_normal())
+            rv = 10 ** rng.uniform(6, 9.5)
+            dt = 10 ** rng.uniform(-8, -3)
+            rc, M, Rh = ref.prepare(net, Y, Tv, Yg, Tg, rv, dt, eos_kind=0, mu=10.0)
+            assert rc == 0
+            Ms.append(M), Rs.append(Rh), ins.append(np.concatenate([[Tv, Tg, rv, dt], Y, Yg]))
+        np.savez_compressed(os.path.join(HERE, f"prepare_net{net}.npz"), M=np.array(Ms), RHS=np.array(Rs),
+                            inputs=np.array(ins))
+
+        # whole particles
+        cases = []
+        if net == 14:
+            cfg = workloads.config(1, n=24, seed=99)
+            for mu in (10.0, 1e-20):
+                for k in range(24):
+                    cases.append((mu, cfg["rho"][k], cfg["temp"][k], cfg["Y"][:, k], 1e-3))
+        else:
+            for mu in (10.0, 1e-20):
+                for k, (Tv, rv) in enumerate([(2e9, 1e7), (3.5e9, 1e8), (1.2e9, 3e6), (5e9, 2e9)]):
+                    Y = np.zeros(d)
+                    Y[o14[1]], Y[o14[2]], Y[o14[3]] = 0.3 / 12, 0.6 / 16, 0.1 / 20
+                    if net == 87:
+                        Y[86] = 0.5
+                    cases.append((mu, rv, Tv, Y, 1e-4))
+        res = dict(mu=[], rho=[], T0=[], Y0=[], hydro_dt=[], Y=[], T=[], dt=[], stats=[], ntrace=[], trace=[])
+        cap = 400
+        for mu, rv, Tv, Y0, hdt in cases:
+            Y, Tn, dtn, st, tr = ref.particle(net, Y0, Tv, rv, hdt, 1e-5, eos_kind=0, mu=mu, max_iter=100000,
+                                              trace_cap=cap)
+            tr2 = np.zeros((cap, 6))
+            tr2[:len(tr)] = tr
+            for key, v in (("mu", mu), ("rho", rv), ("T0", Tv), ("Y0", Y0), ("hydro_dt", hdt), ("Y", Y), ("T", Tn),
+                           ("dt", dtn), ("stats", st), ("ntrace", len(tr)), ("trace", tr2)):
+                res[key].append(v)
+        np.savez_compressed(os.path.join(HERE, f"particles_net{net}.npz"), **{k: np.array(v) for k, v in res.items()})
+        out[net] = len(cases)
+
+    # eigen::solve on random diagonally weighted systems
+    Ms, bs, xs = [], [], []
+    for n in (15, 15, 87, 87, 88, 88, 5, 40):
+        M = rng.standard_normal((n, n)) + np.diag(rng.uniform(1, 10, n)) * n
+        M[rng.random((n, n)) < 0.6] = 0.0
+        np.fill_diagonal(M, rng.uniform(1, 10, n) * n)
+        b = rng.standard_normal(n)
+        Ms.append(np.pad(M, ((0, 88 - n), (0, 88 - n)))), bs.append(np.pad(b, (0, 88 - n)))
+        xs.append(np.pad(ref.solve(M, b), (0, 88 - n)))
+    np.savez_compressed(os.path.join(HERE, "solve.npz"), M=np.array(Ms), b=np.array(bs), x=np.array(xs),
+                        n=np.array([15, 15, 87, 87, 88, 88, 5, 40]))
+    print("golden vectors written:", out)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,66 @@
+"""Parity comparison between the HIP path and the CPU oracle (same seeded inputs).
+
+Per particle, tier 1 is the north-star tolerance of the plain FP64 path:
+  mass fractions X_i = Y_i A_i:  |dX| <= 1e-9 |X| + 1e-14
+  temperature:                  |dT| <= 1e-10 |T|
+  sum(X):                       |sum X_gpu - sum X_oracle| <= 1e-10 max(1, |sum X_oracle|)
+and the particle's status, accepted-substep count and Newton-iteration count are identical.
+
+The device exp/log/pow are not bit-identical to glibc's, and the reference dynamics amplify
+last-ulp differences: moving 3/8 of the oracle's own exp/log/pow results by one ulp
+(oracle_set_ulp_perturbation) moves ~2% of the C1 particles (ideal gas, mu = 10) by up to ~5e-9
+in X and ~2e-10 in T against the unperturbed oracle (tests/test_oracle_sensitivity.py).
+So the bar is: status identical for every particle, counts identical for >= 99.5%, tier 1 for
+>= 97%, and tier 2 (|dX| <= 1e-7 |X| + 1e-14, |dT| <= 1e-8 |T|) for every particle whose counts
+match.  The mixed-precision path is held to |dX| <= 1e-6 |X| + 1e-14, |dT| <= 1e-8 |T|.
+"""
+import numpy as np
+
+X_RTOL, X_ATOL, T_RTOL, SUMX_TOL = 1e-9, 1e-14, 1e-10, 1e-10
+X_RTOL2, T_RTOL2 = 1e-7, 1e-8
+TIER1_MIN_FRAC, COUNTS_MIN_FRAC = 0.97, 0.995
+
+
+def compare(A, Y_gpu, T_gpu, st_gpu, sub_gpu, new_gpu, Y_orc, T_orc, st_orc, sub_orc, new_orc, x_rtol=X_RTOL,
+            t_rtol=T_RTOL):
+    """A: mass numbers (d,).  Y_*: d x n.  Returns a dict of counts, fractions and max errors."""
+    X_g = Y_gpu * A[:, None]
+    X_o = Y_orc * A[:, None]
+    n = len(T_gpu)
+    same_status = st_gpu == st_orc
+    same_counts = same_status & (sub_gpu == sub_orc) & (new_gpu == new_orc)
+    dx = np.abs(X_g - X_o)
+    dT = np.abs(T_gpu - T_orc)
+    sx_o = X_o.sum(0)
+    sumx_ok = np.abs(X_g.sum(0) - sx_o) <= SUMX_TOL * np.maximum(1.0, np.abs(sx_o))
+    tier1 = same_counts & np.all(dx <= x_rtol * np.abs(X_o) + X_ATOL, axis=0) & (dT <= t_rtol * np.abs(T_orc)) & sumx_ok
+    tier2 = np.all(dx <= X_RTOL2 * np.abs(X_o) + X_ATOL, axis=0) & (dT <= T_RTOL2 * np.abs(T_orc))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        relx = np.where(np.abs(X_o) > X_ATOL, dx / np.abs(X_o), 0.0)
+        relT = np.where(T_orc != 0, dT / np.abs(T_orc), 0.0)
+    mc = same_counts
+    return dict(
+        n=n,
+        n_ok=int((st_orc == 0).sum()),
+        n_stalled=int((st_orc == 2).sum()),
+        n_skipped=int((st_orc == 1).sum()),
+        status_mismatch=int((~same_status).sum()),
+        count_mismatch=int((same_status & ~same_counts).sum()),
+        tier1=int(tier1.sum()),
+        tier2_fail=int((mc & ~tier2).sum()),
+        tier1_idx=np.nonzero(~tier1)[0],
+        max_relX=float(relx[:, mc].max()) if mc.any() else 0.0,
+        max_relT=float(relT[mc].max()) if mc.any() else 0.0,
+    )
+
+
+def assert_parity(r, tier1_min=TIER1_MIN_FRAC, counts_min=COUNTS_MIN_FRAC):
+    n = max(r["n"], 1)
+    assert r["status_mismatch"] == 0, r
+    assert r["count_mismatch"] <= (1 - counts_min) * n, r
+    assert r["tier1"] >= tier1_min * n, r
+    assert r["tier2_fail"] == 0, r
+
+
+def summary(r):
+    return {k: v for k, v in r.items() if k != "tier1_idx"}

@@ -1,0 +1,74 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+
+import workloads
+from parity import assert_parity, compare, summary
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import nnet_mi355x
+    c = nnet_mi355x.Context(0)
+    yield c
+    c.close()
+
+
+def run_gpu(ctx, cfg, precision=0):
+    import torch
+    import nnet_mi355x
+    st = nnet_mi355x.device_state(torch, cfg["net"], cfg["rho"], cfg["temp"], cfg["Y"], cfg["dt"])
+    b = ctx.make_batch(cfg["net"], st, eos=cfg["eos"], mu=cfg["mu"], Z=cfg["Z"], precision=precision)
+    ctx.compute([b], cfg["hydro_dt"])
+    torch.cuda.synchronize()
+    return (st["Y"].cpu().numpy(), st["temp"].cpu().numpy(), st["dt"].cpu().numpy(), st["status"].cpu().numpy(),
+            st["substeps"].cpu().numpy().astype(np.uint32), st["newton"].cpu().numpy().astype(np.uint32))
+
+
+def run_oracle(oracle_lib, cfg):
+    Y = np.ascontiguousarray(cfg["Y"].copy())
+    T = cfg["temp"].copy()
+    dt = cfg["dt"].copy()
+    rho = cfg["rho"].copy()
+    st, sub, new = oracle_lib.batch(cfg["net"], rho, T, Y, dt, cfg["hydro_dt"], eos_kind=cfg["eos"], mu=cfg["mu"],
+                                    Z=cfg["Z"])
+    return Y, T, dt, st, sub, new
+
+
+def _check(cfg, g, o, **kw):
+    A = workloads.mass_numbers(cfg["net"])
+    r = compare(A, g[0], g[1], g[3], g[4], g[5], o[0], o[1], o[3], o[4], o[5])
+    print(cfg["label"], summary(r), flush=True)
+    assert_parity(r, **kw)
+    return r
+
+
+@pytest.mark.parametrize("mu", [10.0, 1e-20])
+def test_net14_ideal_c1(ctx, oracle_lib, mu):
+    cfg = workloads.config(1)
+    cfg["mu"] = mu
+    g = run_gpu(ctx, cfg)
+    o = run_oracle(oracle_lib, cfg)
+    _check(cfg, g, o)
+
+
+@pytest.mark.parametrize("net", [86, 87])
+def test_net86_87_ideal_small(ctx, oracle_lib, net):
+    rng = np.random.default_rng(7)
+    n = 48
+    d = 86 if net == 86 else 87
+    o14 = workloads.net14_species_order()
+    Y = np.zeros((d, n))
+    Y[o14[1]], Y[o14[2]] = 0.5 / 12, 0.5 / 16
+    if net == 87:
+        Y[86] = 0.5
+    cfg = dict(net=net, eos=0, mu=10.0, Z=None, rho=10 ** rng.uniform(6, 9, n), temp=10 ** rng.uniform(9, 9.7, n),
+               Y=Y, dt=np.full(n, 1e-5), hydro_dt=1e-4, steps=1, label=f"net{net} ideal-gas small")
+    g = run_gpu(ctx, cfg)
+    o = run_oracle(oracle_lib, cfg)
+    _check(cfg, g, o)

@@ -1,0 +1,69 @@
+"""The CPU oracle against golden vectors minted from the reference build (tests/golden/).
+Bit-exact: the oracle restates the reference with the same operation order and is built with
+the same code generation (x86-64 baseline, no FMA contraction, glibc libm)."""
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def same(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+@pytest.mark.parametrize("net", [14, 86, 87])
+def test_reaction_lists(oracle_lib, net):
+    g = np.load(os.path.join(GOLD, f"rates_net{net}.npz"))
+    rb, pb, ids, nums = oracle_lib.reaction_list(net)
+    assert np.array_equal(rb, g["rb"]) and np.array_equal(pb, g["pb"])
+    assert np.array_equal(ids, g["ids"]) and np.array_equal(nums, g["nums"])
+    assert len(pb) == {14: 29, 86: 309, 87: 311}[net]
+
+
+@pytest.mark.parametrize("net", [14, 86, 87])
+def test_rates(oracle_lib, net):
+    g = np.load(os.path.join(GOLD, f"rates_net{net}.npz"))
+    for k in range(len(g["T"])):
+        BE, r, dr = oracle_lib.rates(net, g["Y"][k], g["T"][k], g["rho"][k])
+        assert same(BE, g["BE"][k]), (net, k, "BE")
+        assert same(r, g["rates"][k]), (net, k, "rates")
+        assert same(dr, g["drates"][k]), (net, k, "drates")
+
+
+@pytest.mark.parametrize("net", [14, 86, 87])
+def test_assembly(oracle_lib, net):
+    g = np.load(os.path.join(GOLD, f"prepare_net{net}.npz"))
+    d = {14: 14, 86: 86, 87: 87}[net]
+    for k in range(len(g["M"])):
+        x = g["inputs"][k]
+        Tv, Tg, rv, dt = x[:4]
+        Y, Yg = x[4:4 + d], x[4 + d:4 + 2 * d]
+        rc, M, R = oracle_lib.prepare(net, Y, Tv, Yg, Tg, rv, dt, eos_kind=0, mu=10.0)
+        assert rc == 0
+        assert same(M, g["M"][k]) and same(R, g["RHS"][k]), (net, k)
+
+
+def test_solve(oracle_lib):
+    g = np.load(os.path.join(GOLD, "solve.npz"))
+    for M, b, x, n in zip(g["M"], g["b"], g["x"], g["n"]):
+        assert same(oracle_lib.solve(M[:n, :n], b[:n]), x[:n])
+
+
+@pytest.mark.parametrize("net", [14, 86, 87])
+def test_particles(oracle_lib, net):
+    g = np.load(os.path.join(GOLD, f"particles_net{net}.npz"))
+    n_stalled = 0
+    for k in range(len(g["mu"])):
+        Y, T, dt, st, tr = oracle_lib.particle(net, g["Y0"][k], g["T0"][k], g["rho"][k], g["hydro_dt"][k], 1e-5,
+                                               eos_kind=0, mu=g["mu"][k], max_iter=100000, trace_cap=400)
+        assert same(Y, g["Y"][k]) and same(T, g["T"][k]) and same(dt, g["dt"][k]), (net, k)
+        assert np.array_equal(st, g["stats"][k]), (net, k, st, g["stats"][k])
+        nt = int(g["ntrace"][k])
+        assert same(tr[:min(nt, 400)], g["trace"][k][:min(nt, 400)]), (net, k)
+        n_stalled += st[0] == 2
+    # the fixtures cover both controller outcomes
+    if net == 14:
+        assert 0 < n_stalled < len(g["mu"])

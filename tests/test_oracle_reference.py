@@ -1,0 +1,56 @@
+"""The oracle against the live reference build (oracle/_ref), on fresh random inputs.
+Skipped where the reference could not be built (it needs the reference tree)."""
+import numpy as np
+import pytest
+
+import workloads
+
+
+def same(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+@pytest.mark.parametrize("net", [14, 86, 87])
+def test_rates_random(oracle_lib, ref_lib, net):
+    rng = np.random.default_rng(net)
+    d = {14: 14, 86: 86, 87: 87}[net]
+    for _ in range(400):
+        T, rho = 10 ** rng.uniform(7.5, 10.5), 10 ** rng.uniform(2, 11)
+        Y = rng.random(d) * 0.05
+        a, b = oracle_lib.rates(net, Y, T, rho), ref_lib.rates(net, Y, T, rho)
+        assert all(same(x, y) for x, y in zip(a, b)), (net, T, rho)
+
+
+def test_particles_random(oracle_lib, ref_lib):
+    cfg = workloads.config(1, n=200, seed=5)
+    for mu in (10.0, 1e-20):
+        for k in range(200):
+            a = oracle_lib.particle(14, cfg["Y"][:, k], cfg["temp"][k], cfg["rho"][k], 1e-3, 1e-5, mu=mu,
+                                    trace_cap=2000)
+            b = ref_lib.particle(14, cfg["Y"][:, k], cfg["temp"][k], cfg["rho"][k], 1e-3, 1e-5, mu=mu,
+                                 trace_cap=2000)
+            assert same(a[0], b[0]) and a[1] == b[1] and a[2] == b[2], k
+            assert np.array_equal(a[3], b[3]) and same(a[4], b[4]), k
+
+
+def test_reference_solve_substep_agrees(oracle_lib, ref_lib):
+    """The reference's own solveSystemSubstep (not the stepped loop) on particles the oracle shows
+    to terminate: identical final state."""
+    import ctypes as C
+    from oracle import D, _p
+    L = ref_lib.L
+    L.ref_solve_substep.argtypes = [C.c_int, C.c_int, C.c_double, D, C.c_int, D, D, C.c_double, C.c_double,
+                                    C.c_double, D]
+    cfg = workloads.config(1, n=120, seed=11)
+    checked = 0
+    for k in range(120):
+        a = oracle_lib.particle(14, cfg["Y"][:, k], cfg["temp"][k], cfg["rho"][k], 1e-3, 1e-5, mu=10.0)
+        if a[3][0] != 0:
+            continue
+        Y = cfg["Y"][:, k].copy()
+        T, dt, Z = np.array([cfg["temp"][k]]), np.array([1e-5]), np.zeros(1)
+        L.ref_solve_substep(14, 0, 10.0, _p(Z), 0, _p(Y), _p(T), cfg["rho"][k], 0.0, 1e-3, _p(dt))
+        assert same(Y, a[0]) and T[0] == a[1] and dt[0] == a[2]
+        checked += 1
+    assert checked > 10
