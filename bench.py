@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Throughput of the batched implicit nuclear-network solve on one MI355X.
+
+Workload (default): net86 + Helmholtz EOS, 2,097,152 particles, 70% cool (T log-U[1e8,1e9] K,
+rho log-U[1e4,1e7] g/cm^3) / 30% hot (T log-U[3e9,8e9], rho log-U[1e7,5e9]), X(12C)=0.3,
+X(16O)=0.6, X(20Ne)=0.1, hydro dt = 1e-4 s per step, state resident on the device.  A "step" is
+one computeNuclearReactions call over all particles (one hydro step).
+
+Prints one JSON line: particle-steps/s over the timed steps, the dominant kernel's roofline
+(algorithmic FLOPs from the device Newton-iteration counters: per iteration of an n = d+1
+system, 2n^3/3 + 1.5n^2 - 7n/6 (dense unpivoted LU + solve) + 3d^2 (assembly), divided by the
+kernel time measured with HIP events on the stream the kernel runs on), and the CPU baseline:
+the FP64 CPU restatement of the reference's OpenMP path timed on this host on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "sph-exa-nnet_amd")]
+
+import workloads  # noqa: E402
+
+TABLE = os.path.join(ROOT, "sph-exa-nnet_amd", "data", "helm_table.bin")
+PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix)
+
+
+def flops_per_iter(net):
+    d = {14: 14, 86: 86, 87: 87}[net]
+    n = d + 1
+    return 2 * n ** 3 / 3 + 1.5 * n ** 2 - 7 * n / 6 + 3 * d ** 2
+
+
+def device_config(torch, cid, n):
+    """Upload the seeded particle set of configuration cid (Y built on the device)."""
+    cfg = workloads.config(cid, n=n, seed=workloads.SEED0 + cid)
+    dev = "cuda:0"
+    st = {
+        "rho": torch.tensor(cfg["rho"], dtype=torch.float64, device=dev),
+        "rho_m1": torch.tensor(cfg["rho"], dtype=torch.float64, device=dev),
+        "temp": torch.tensor(cfg["temp"], dtype=torch.float64, device=dev),
+        "dt": torch.tensor(cfg["dt"], dtype=torch.float64, device=dev),
+        "status": torch.zeros(n, dtype=torch.uint8, device=dev),
+        "substeps": torch.zeros(n, dtype=torch.int32, device=dev),
+        "newton": torch.zeros(n, dtype=torch.int32, device=dev),
+    }
+    Y = torch.empty(cfg["Y"].shape, dtype=torch.float64, device=dev)
+    for s in range(cfg["Y"].shape[0]):
+        Y[s].copy_(torch.from_numpy(cfg["Y"][s]))
+    st["Y"] = Y
+    return cfg, st
+
+
+def cpu_baseline(cid, sample, steps_hint):
+    """The CPU restatement of the reference OpenMP path (oracle) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    o = Oracle()
+    o.helm_load(TABLE)
+    cfg = workloads.config(cid, n=sample, seed=workloads.SEED0 + 100 + cid)
+    Y = np.ascontiguousarray(cfg["Y"])
+    T, dt, rho = cfg["temp"].copy(), cfg["dt"].copy(), cfg["rho"].copy()
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    st, sub, new = o.batch(cfg["net"], rho, T, Y, dt, cfg["hydro_dt"], eos_kind=cfg["eos"], mu=cfg["mu"],
+                           Z=cfg["Z"], nthreads=threads)
+    el = time.perf_counter() - t0
+    return {"value": sample / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"first hydro step of {sample} particles of the same distribution (seed +100), "
+                      f"{el:.2f} s, statuses {np.bincount(st, minlength=5).tolist()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=3, choices=[2, 3, 4])
+    ap.add_argument("--n", type=int, default=None, help="particles (default: the configuration's size)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--cpu-sample", type=int, default=8192)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(particles are independent: N GPUs would each run their own share)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import nnet_mi355x as nm
+
+    n = args.n or {2: 1048576, 3: 2097152, 4: 2097152}[args.config]
+    cfg, st = device_config(torch, args.config, n)
+    ctx = nm.Context(0, helm_table=TABLE)
+    prec = nm.PRECISION_FP64 if args.precision == "fp64" else nm.PRECISION_MIXED
+    batch = ctx.make_batch(cfg["net"], st, eos=cfg["eos"], mu=cfg["mu"], Z=cfg["Z"], precision=prec)
+    stream = torch.cuda.current_stream()
+    hdt = cfg["hydro_dt"]
+    print(f"# {cfg['label']}: n={n} precision={args.precision} warmup={args.warmup} steps={args.steps}",
+          file=sys.stderr, flush=True)
+
+    for w in range(args.warmup):
+        ctx.compute([batch], hdt, stream=stream.cuda_stream, sync=False)
+        torch.cuda.synchronize()
+        print(f"# warmup step {w} done", file=sys.stderr, flush=True)
+
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    newton_total = 0
+    status_hist = np.zeros(5, np.int64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        ctx.compute([batch], hdt, stream=stream.cuda_stream, sync=False)
+        ev[k][1].record(stream)
+        # counters of this step (tiny D2H reads after the kernel on the same stream)
+        newton_total += int(st["newton"].sum().item())
+        status_hist += np.bincount(st["status"].cpu().numpy(), minlength=5)
+        print(f"# step {k} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [a.elapsed_time(b) for a, b in ev]
+    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
+
+    value = n * args.steps / wall
+    flops_per_launch = newton_total / args.steps * flops_per_iter(cfg["net"])
+    achieved = flops_per_launch / avg_kernel_s
+    out = {
+        "metric": "net86 particle-steps/s, mixed vs FP64 on 1 GPU at <=1e-6 rel mass-frac error",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": cfg["label"], "particles": n, "hydro_dt": hdt, "precision": args.precision,
+                   "newton_iters_per_step": newton_total / args.steps,
+                   "status_per_step": (status_hist / args.steps).tolist()},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64, "traffic": None},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_sample, args.steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

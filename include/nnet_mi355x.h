@@ -114,7 +114,10 @@ typedef struct nnet_batch {
 int nnet_compute(nnet_ctx* ctx, const nnet_batch* batches, int nb, double hydro_dt,
                  double previous_dt, int use_drhodt, void* stream);
 
-/* Cap on Newton iterations per particle per call (default 200000; 0 = no cap). */
+/* Cap on Newton iterations per particle per call (default 10000; 0 = no cap).  The reference
+ * controller can also cycle between two rejected timesteps for thousands of attempts before (or
+ * instead of) accepting one; the cap bounds that work.  A capped particle is left at its last
+ * accepted substep with status NNET_STATUS_MAX_ITER. */
 int nnet_ctx_set_max_iterations(nnet_ctx* ctx, long max_iter);
 
 /* Blocking variant of nnet_compute (drop-in for the reference's synchronous launch). */

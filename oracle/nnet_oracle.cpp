@@ -316,7 +316,7 @@ struct Work
 
 struct ParticleStats
 {
-    uint32_t substeps = 0, newton = 0, rejections = 0;
+    uint32_t substeps = 0, newton = 0, rejections = 0, max_run = 0, run = 0;
     uint8_t  status   = ST_OK;
 };
 
@@ -453,11 +453,14 @@ static ParticleStats solve_particle(int net, int d, const EosSpec& eosSpec, Work
             i = 0;
             elapsed += timestep;
             ++st.substeps;
+            st.run = 0;
             if ((dt_tot - elapsed) / dt_tot < k::substep_dtTol) return st;
         }
         else if (i == 0)
         {
             ++st.rejections;
+            ++st.run;
+            if (st.run > st.max_run) st.max_run = st.run;
             // A rejected attempt restarts from the same accepted state; when the new attempt's
             // dt and Newton seed are bit-identical to this one's, it is replayed forever.
             if (same_bits(dt, att_dt) && same_bits(T_buf, att_Tbuf))
@@ -548,6 +551,46 @@ extern "C"
 
     void oracle_solve(double* M, double* RHS, double* X, int n) { gauss_solve(M, RHS, X, n, k::epsilonSystem); }
 
+    // The biquintic interpolation of the loaded table alone (helmholtz.cpp:357-557 for ye = 1):
+    // out = free, df_d, df_t, df_tt, df_dt at (din, T).  Used to pin the interpolation.
+    void oracle_helm_interp(double din, double temp, double* out)
+    {
+        const helm::Table& tb = g_helm;
+        using namespace helm;
+        int jat = int((std::log10(temp) - tlo) * tstpi);
+        jat     = std::max<int>(1, std::min<int>(jat, jmax - 2));
+        int iat = int((std::log10(din) - dlo) * dstpi);
+        iat     = std::max<int>(1, std::min<int>(iat, imax - 2));
+        double    fi[36];
+        const int ks5[9] = {F, FT, FTT, FD, FDD, FDT, FDDT, FDTT, FDDTT};
+        for (int q = 0; q < 9; ++q)
+        {
+            fi[4 * q + 0] = tb.at(ks5[q], iat + 0, jat + 0);
+            fi[4 * q + 1] = tb.at(ks5[q], iat + 1, jat + 0);
+            fi[4 * q + 2] = tb.at(ks5[q], iat + 0, jat + 1);
+            fi[4 * q + 3] = tb.at(ks5[q], iat + 1, jat + 1);
+        }
+        const double xt = std::max<double>((temp - tb.t_[jat]) * tb.dti_sav[jat], 0.);
+        const double xd = std::max<double>((din - tb.d[iat]) * tb.ddi_sav[iat], 0.);
+        const double mxt = 1. - xt, mxd = 1. - xd;
+        const double si0t = psi0(xt), si1t = psi1(xt) * tb.dt_sav[jat], si2t = psi2(xt) * tb.dt2_sav[jat];
+        const double si0mt = psi0(mxt), si1mt = -psi1(mxt) * tb.dt_sav[jat], si2mt = psi2(mxt) * tb.dt2_sav[jat];
+        const double si0d = psi0(xd), si1d = psi1(xd) * tb.dd_sav[iat], si2d = psi2(xd) * tb.dd2_sav[iat];
+        const double si0md = psi0(mxd), si1md = -psi1(mxd) * tb.dd_sav[iat], si2md = psi2(mxd) * tb.dd2_sav[iat];
+        const double dsi0t = dpsi0(xt) * tb.dti_sav[jat], dsi1t = dpsi1(xt), dsi2t = dpsi2(xt) * tb.dt_sav[jat];
+        const double dsi0mt = -dpsi0(mxt) * tb.dti_sav[jat], dsi1mt = dpsi1(mxt), dsi2mt = -dpsi2(mxt) * tb.dt_sav[jat];
+        const double dsi0d = dpsi0(xd) * tb.ddi_sav[iat], dsi1d = dpsi1(xd), dsi2d = dpsi2(xd) * tb.dd_sav[iat];
+        const double dsi0md = -dpsi0(mxd) * tb.ddi_sav[iat], dsi1md = dpsi1(mxd), dsi2md = -dpsi2(mxd) * tb.dd_sav[iat];
+        const double ddsi0t = ddpsi0(xt) * tb.dt2i_sav[jat], ddsi1t = ddpsi1(xt) * tb.dti_sav[jat], ddsi2t = ddpsi2(xt);
+        const double ddsi0mt = ddpsi0(mxt) * tb.dt2i_sav[jat], ddsi1mt = -ddpsi1(mxt) * tb.dti_sav[jat],
+                     ddsi2mt = ddpsi2(mxt);
+        out[0] = h5(fi, si0t, si1t, si2t, si0mt, si1mt, si2mt, si0d, si1d, si2d, si0md, si1md, si2md);
+        out[1] = h5(fi, si0t, si1t, si2t, si0mt, si1mt, si2mt, dsi0d, dsi1d, dsi2d, dsi0md, dsi1md, dsi2md);
+        out[2] = h5(fi, dsi0t, dsi1t, dsi2t, dsi0mt, dsi1mt, dsi2mt, si0d, si1d, si2d, si0md, si1md, si2md);
+        out[3] = h5(fi, ddsi0t, ddsi1t, ddsi2t, ddsi0mt, ddsi1mt, ddsi2mt, si0d, si1d, si2d, si0md, si1md, si2md);
+        out[4] = h5(fi, dsi0t, dsi1t, dsi2t, dsi0mt, dsi1mt, dsi2mt, dsi0d, dsi1d, dsi2d, dsi0md, dsi1md, dsi2md);
+    }
+
     // One assembly (prepareSystemNR-style inputs are given explicitly).
     int oracle_prepare(int net, int eos_kind, double mu, const double* Z, int zdim, const double* Y, double T,
                        const double* Yg, double Tg, double rho, double drho_dt, double dt, double* Mp, double* RHS)
@@ -578,6 +621,7 @@ extern "C"
         stats[1]           = st.substeps;
         stats[2]           = st.newton;
         stats[3]           = st.rejections;
+        stats[4]           = st.max_run;
         return tr.len;
     }
 
@@ -587,7 +631,7 @@ extern "C"
     void oracle_batch(int net, int eos_kind, double mu, const double* Z, int zdim, size_t n, double* rho,
                       double* rho_m1, double** Y, double* temp, double* dt, double hydro_dt, double previous_dt,
                       int use_drhodt, uint8_t* status, uint32_t* substeps, uint32_t* newton, int nthreads,
-                      long max_iter)
+                      long max_iter, uint32_t* max_run)
     {
         const int d  = dimension_of(net);
         const int nR = reactions_for(net).size();
@@ -627,6 +671,7 @@ extern "C"
                 if (status) status[i] = st.status;
                 if (substeps) substeps[i] = st.substeps;
                 if (newton) newton[i] = st.newton;
+                if (max_run) max_run[i] = st.max_run;
             }
         }
     }

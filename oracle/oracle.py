@@ -90,12 +90,13 @@ class _Lib:
                                 _p(np.ascontiguousarray(Yg, dtype=np.float64)), Tg, rho, drho_dt, dt, _p(M), _p(R))
         return rc, M.reshape(n, n, order="F"), R
 
-    def particle(self, net, Y, T, rho, dt_tot, dt, eos_kind=0, mu=10.0, Z=None, max_iter=200000, trace_cap=0,
+    def particle(self, net, Y, T, rho, dt_tot, dt, eos_kind=0, mu=10.0, Z=None, max_iter=10000, trace_cap=0,
                  drho_dt=0.0):
-        """One particle through the substep solver.  Returns (Y, T, dt, stats[4], trace)."""
+        """One particle through the substep solver.  Returns (Y, T, dt, stats, trace); stats = status,
+        accepted substeps, Newton iterations, rejected attempts, longest run of rejected attempts."""
         Y = np.array(Y, dtype=np.float64)
         Tt, dtt = np.array([T], np.float64), np.array([dt], np.float64)
-        st = np.zeros(4)
+        st = np.zeros(5)
         tr = np.zeros(6 * max(trace_cap, 1))
         Zc = np.ascontiguousarray(Z if Z is not None else np.zeros(1), dtype=np.float64)
         n = self.fn("particle")(net, eos_kind, mu, _p(Zc), len(Zc) if Z is not None else 0, _p(Y), _p(Tt), rho,
@@ -111,7 +112,7 @@ class Oracle(_Lib):
         L.oracle_helm_raw.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, D]
         L.oracle_batch.argtypes = [C.c_int, C.c_int, C.c_double, D, C.c_int, C.c_size_t, D, D, C.POINTER(D), D, D,
                                    C.c_double, C.c_double, C.c_int, C.POINTER(C.c_uint8), C.POINTER(C.c_uint32),
-                                   C.POINTER(C.c_uint32), C.c_int, C.c_long]
+                                   C.POINTER(C.c_uint32), C.c_int, C.c_long, C.POINTER(C.c_uint32)]
         L.oracle_dynamic_batch_size.argtypes = [C.c_size_t, C.c_int]
 
     def helm_load(self, path):
@@ -125,9 +126,10 @@ class Oracle(_Lib):
         return out
 
     def batch(self, net, rho, temp, Y, dt, hydro_dt, previous_dt=None, eos_kind=0, mu=10.0, Z=None, rho_m1=None,
-              use_drhodt=False, nthreads=0, max_iter=200000):
+              use_drhodt=False, nthreads=0, max_iter=10000):
         """In-place batch integration of numpy arrays (Y: d x n, C-contiguous rows).
-        Returns (status, substeps, newton)."""
+        Returns (status, substeps, newton); self.last_max_run holds, per particle, the longest run
+        of consecutive rejected attempts."""
         n = len(rho)
         d = DIMS[net]
         assert Y.shape == (d, n) and Y.flags["C_CONTIGUOUS"] and Y.dtype == np.float64
@@ -135,13 +137,15 @@ class Oracle(_Lib):
         status = np.zeros(n, np.uint8)
         sub = np.zeros(n, np.uint32)
         newt = np.zeros(n, np.uint32)
+        mrun = np.zeros(n, np.uint32)
         rm1 = rho_m1 if rho_m1 is not None else rho.copy()
         Zc = np.ascontiguousarray(Z if Z is not None else np.zeros(1), dtype=np.float64)
         self.L.oracle_batch(net, eos_kind, mu, _p(Zc), len(Zc) if Z is not None else 0, n, _p(rho), _p(rm1), rows,
                             _p(temp), _p(dt), hydro_dt, hydro_dt if previous_dt is None else previous_dt,
                             int(use_drhodt), status.ctypes.data_as(C.POINTER(C.c_uint8)),
                             sub.ctypes.data_as(C.POINTER(C.c_uint32)), newt.ctypes.data_as(C.POINTER(C.c_uint32)),
-                            nthreads, max_iter)
+                            nthreads, max_iter, mrun.ctypes.data_as(C.POINTER(C.c_uint32)))
+        self.last_max_run = mrun
         return status, sub, newt
 
 

@@ -809,7 +809,7 @@ struct nnet_ctx
 {
     int         device = 0;
     std::string err;
-    long        max_iter = 200000;
+    long        max_iter = 10000;
     // Helmholtz
     HelmGrid* d_grid  = nullptr;
     double*   d_cells = nullptr;

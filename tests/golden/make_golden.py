@@ -115,6 +115,48 @@ def main():
     np.savez_compressed(os.path.join(HERE, "solve.npz"), M=np.array(Ms), b=np.array(bs), x=np.array(xs),
                         n=np.array([15, 15, 87, 87, 88, 88, 5, 40]))
     print("golden vectors written:", out)
+    if ref.has_helmholtz:
+        make_golden_helm(ref)
+
+
+def table_digest(path):
+    import hashlib
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def make_golden_helm(ref):
+    """Helmholtz EOS outputs and whole particles (C2/C3/C4 samples) from the reference compiled
+    against the generated table; the table's digest is stored so stale fixtures are detected."""
+    table = os.path.join(ROOT, "sph-exa-nnet_amd", "data", "helm_table.bin")
+    rng = np.random.default_rng(777)
+    pts, outs, nets = [], [], []
+    for k in range(60):
+        net = (14, 86, 87)[k % 3]
+        Z = workloads.helmholtz_Z(net)
+        Y = rng.random(len(workloads.mass_numbers(net))) * 0.03
+        T, rho = 10 ** rng.uniform(7.5, 10.2), 10 ** rng.uniform(2, 11)
+        outs.append(ref.eos(1, Y, T, rho, Z=Z))
+        pts.append(np.concatenate([[T, rho], np.pad(Y, (0, 87 - len(Y)))]))
+        nets.append(net)
+    np.savez_compressed(os.path.join(HERE, "helm_eos.npz"), pts=np.array(pts), out=np.array(outs),
+                        net=np.array(nets), digest=table_digest(table))
+    for cid, n in ((2, 40), (3, 16), (4, 8)):
+        cfg = workloads.config(cid, n=n, seed=31)
+        net, d = cfg["net"], cfg["Y"].shape[0]
+        res = dict(rho=[], T0=[], Y0=[], Y=[], T=[], dt=[], stats=[])
+        for k in range(n):
+            if not (cfg["rho"][k] > 1e5 and cfg["temp"][k] > 1e8):
+                continue
+            Y, Tn, dtn, st, _ = ref.particle(net, cfg["Y"][:, k], cfg["temp"][k], cfg["rho"][k], cfg["hydro_dt"], 1e-5,
+                                             eos_kind=1, Z=cfg["Z"], max_iter=10000)
+            for key, v in (("rho", cfg["rho"][k]), ("T0", cfg["temp"][k]), ("Y0", cfg["Y"][:, k]), ("Y", Y),
+                           ("T", Tn), ("dt", dtn), ("stats", st)):
+                res[key].append(v)
+        np.savez_compressed(os.path.join(HERE, f"helm_particles_c{cid}.npz"), Z=cfg["Z"], net=net,
+                            hydro_dt=cfg["hydro_dt"], digest=table_digest(table),
+                            **{k: np.array(v) for k, v in res.items()})
+    print("Helmholtz golden vectors written")
 
 
 if __name__ == "__main__":

@@ -8,13 +8,19 @@ from parity import assert_parity, compare, summary
 pytestmark = pytest.mark.gpu
 
 
+import os
+
+TABLE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sph-exa-nnet_amd", "data",
+                     "helm_table.bin")
+
+
 @pytest.fixture(scope="module")
 def ctx():
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
     import nnet_mi355x
-    c = nnet_mi355x.Context(0)
+    c = nnet_mi355x.Context(0, helm_table=TABLE if os.path.exists(TABLE) else None)
     yield c
     c.close()
 
@@ -30,21 +36,31 @@ def run_gpu(ctx, cfg, precision=0):
             st["substeps"].cpu().numpy().astype(np.uint32), st["newton"].cpu().numpy().astype(np.uint32))
 
 
-def run_oracle(oracle_lib, cfg):
+def run_oracle(oracle_lib, cfg, perturb=False):
+    if cfg["eos"] == 1:
+        oracle_lib.helm_load(TABLE)
+    oracle_lib.L.oracle_set_ulp_perturbation(1 if perturb else 0)
     Y = np.ascontiguousarray(cfg["Y"].copy())
     T = cfg["temp"].copy()
     dt = cfg["dt"].copy()
     rho = cfg["rho"].copy()
     st, sub, new = oracle_lib.batch(cfg["net"], rho, T, Y, dt, cfg["hydro_dt"], eos_kind=cfg["eos"], mu=cfg["mu"],
                                     Z=cfg["Z"])
-    return Y, T, dt, st, sub, new
+    oracle_lib.L.oracle_set_ulp_perturbation(0)
+    return Y, T, dt, st, sub, new, oracle_lib.last_max_run.copy()
 
 
-def _check(cfg, g, o, **kw):
+def _check(cfg, g, o, oracle_lib=None, calibrated=True):
+    """GPU vs oracle; with calibrated=True the bar is set by the 1-ulp-perturbed oracle."""
     A = workloads.mass_numbers(cfg["net"])
-    r = compare(A, g[0], g[1], g[3], g[4], g[5], o[0], o[1], o[3], o[4], o[5])
-    print(cfg["label"], summary(r), flush=True)
-    assert_parity(r, **kw)
+    r = compare(A, g[0], g[1], g[3], g[4], g[5], o[0], o[1], o[3], o[4], o[5], max_run_orc=o[6])
+    print(cfg["label"], "gpu vs oracle:", summary(r), flush=True)
+    rp = None
+    if calibrated:
+        p = run_oracle(oracle_lib, cfg, perturb=True)
+        rp = compare(A, p[0], p[1], p[3], p[4], p[5], o[0], o[1], o[3], o[4], o[5], max_run_orc=o[6])
+        print(cfg["label"], "perturbed oracle vs oracle:", summary(rp), flush=True)
+    assert_parity(r, rp)
     return r
 
 
@@ -54,13 +70,13 @@ def test_net14_ideal_c1(ctx, oracle_lib, mu):
     cfg["mu"] = mu
     g = run_gpu(ctx, cfg)
     o = run_oracle(oracle_lib, cfg)
-    _check(cfg, g, o)
+    _check(cfg, g, o, oracle_lib)
 
 
 @pytest.mark.parametrize("net", [86, 87])
 def test_net86_87_ideal_small(ctx, oracle_lib, net):
     rng = np.random.default_rng(7)
-    n = 48
+    n = 96
     d = 86 if net == 86 else 87
     o14 = workloads.net14_species_order()
     Y = np.zeros((d, n))
@@ -71,4 +87,13 @@ def test_net86_87_ideal_small(ctx, oracle_lib, net):
                Y=Y, dt=np.full(n, 1e-5), hydro_dt=1e-4, steps=1, label=f"net{net} ideal-gas small")
     g = run_gpu(ctx, cfg)
     o = run_oracle(oracle_lib, cfg)
-    _check(cfg, g, o)
+    _check(cfg, g, o, oracle_lib)
+
+
+@pytest.mark.skipif(not os.path.exists(TABLE), reason="Helmholtz table not generated")
+@pytest.mark.parametrize("cid,n", [(2, 4096), (3, 512), (4, 48)])
+def test_helmholtz_configs(ctx, oracle_lib, cid, n):
+    cfg = workloads.config(cid, n=n, seed=1000 + cid)
+    g = run_gpu(ctx, cfg)
+    o = run_oracle(oracle_lib, cfg)
+    _check(cfg, g, o, oracle_lib)
