@@ -17,6 +17,7 @@
 // different stiffness across the CUs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -48,44 +49,42 @@ constexpr int MAXQ = 3; // reactant entries per reaction
 // ---------------------------------------------------------------------------------------------
 struct NetDev
 {
-    int net, d, nR, nE, nfit, fit_e0, rev_div;
+    int        net, d, nR, nE, nfit, fit_e0, rev_div, nnz;
     SpecialIdx sp;
-    // rates
-    const double* BE0;     // [d'] constant binding energies (erg/mol), d' = 14 or 86
-    const double* Zr;      // [d'] charges used by the rate functor
-    const double* Z53;     // [d'] Z^(5/3) (host glibc pow, the value the reference computes)
-    const double* fit;     // [nfit][8]
-    const double* q;       // [nfit]
-    const double* choose;  // [rows][24]
-    const int*    ca;      // [nfit] choose row of the reverse numerator
-    const int*    cb;      // [nfit] choose row of the reverse denominator
-    const int*    photo;   // [nfit] reverse is a photodisintegration (carries T^3/2)
-    const int*    scr;     // [nE][4]: mu index a, b, c, e (e < 0: none)
-    const int*    swap;    // [nE]
-    const int*    scr_l;   // [nE] reverse rate is screened too
-    const int*    push_e;  // [nE] rate index of eff[i] (-1: not in the list)
-    const int*    push_l;  // [nE] rate index of l[i]
-    const double* elogT;   // net87 electron table
+    // rate parameterization
+    const double* BE0;    // [d'] constant binding energies (erg/mol), d' = 14 or 86
+    const double* Z53;    // [d'] Z^(5/3) (host glibc pow: the value the reference computes)
+    const double* fit;    // [nfit][8]
+    const double* q;      // [nfit]
+    const double* choose; // [rows][24]
+    const int*    ca;     // [nfit] choose row of the reverse numerator
+    const int*    cb;     // [nfit] choose row of the reverse denominator
+    const int*    photo;  // [nfit] reverse is a photodisintegration (carries T^3/2)
+    const int*    scr;    // [nE][4]: mu index a, b, c, e (e < 0: none)
+    const int*    swap;   // [nE]
+    const int*    scr_l;  // [nE] reverse rate is screened too
+    const int*    push_e; // [nE] rate index of eff[i] (-1: not in the list)
+    const int*    push_l; // [nE] rate index of l[i]
+    const double* elogT;  // net87 electron-capture table
     const double* elogR;
     const double* etab;
-    double        BEe;     // net87 electron binding energy (erg/mol)
-    // reactions
-    const int* rb;   // [nR+1] reactant begin
-    const int* pb;   // [nR] product begin
-    const int* ids;  // species id of entry
-    const int* nums; // count of entry
-    // derivativesFromReactions: species s -> ordered (reaction, sign, count)
-    const int* sp_ptr;
-    const int* sp_r;
-    const int* sp_c; // signed count (+ produced / - consumed)
-    // firstOrderDYFromReactions: dense entry a*d+b -> ordered (reaction, reactant pos, signed count)
-    const int* jp_ptr;
-    const int* j_r;
-    const int* j_q;
-    const int* j_c;
-    // rows of the RHS correction: row a -> ascending b with a structural entry
-    const int* row_ptr;
-    const int* row_b;
+    double        BEe; // net87 electron binding energy (erg/mol)
+    // Assembly image, copied into LDS once per workgroup (32-bit words; sub-arrays at the
+    // offsets below, in units of their element size):
+    //   rx[nR+1]      word: reactant_begin | product_begin << 16 (rx[nR]: entry count)
+    //   ent16[]       halfword: species | count << 8 (reference entry order)
+    //   sp_ptr16[d+1] halfword: species s sums entries [sp_ptr[s], sp_ptr[s+1]) of sp_ent
+    //   sp_ent[]      word: reaction | (signed count & 0xff) << 16, in reaction order
+    //   nz_ptr16[nnz+1], nz_ent[]: Jacobian non-zero e (row-major) sums (reaction, reactant
+    //                 position, signed count): r | q << 12 | (c & 0xff) << 16, in the order
+    //                 of the reference's scatter loops
+    //   row_ptr16[d+1], nz_col8[nnz]: non-zeros of row a are [row_ptr[a], row_ptr[a+1]), columns
+    //                 ascending
+    const uint32_t* img;
+    int             img_words, oRx, oEnt16, oSpPtr16, oSpEnt, oNzPtr16, oNzEnt, oRowPtr16, oNzCol8;
+    // per lane, per register slot: index of the Jacobian non-zero the slot holds (-1: zero),
+    // laid out [slot][lane] for coalesced loads
+    const int16_t* slotmap;
 };
 
 constexpr int MAXSP = 88; // species + 1 of the largest network
@@ -103,15 +102,16 @@ struct EosDev
 
 struct BatchDev
 {
-    size_t    n;
+    size_t        n;
     const double* rho;
-    double*   rho_m1;
-    double*   temp;
-    double*   dt;
-    uint8_t*  status;
-    uint32_t* substeps;
-    uint32_t* newton;
-    double*   Y[MAXSP]; // per-species abundance arrays (SoA)
+    double*       rho_m1;
+    double*       temp;
+    double*       dt;
+    uint8_t*      status;
+    uint32_t*     substeps;
+    uint32_t*     newton;
+    unsigned int* queue; // work-queue counter (zeroed by nnet_queue_reset before the launch)
+    double*       Y[MAXSP]; // per-species abundance arrays (SoA)
 };
 
 struct StepParams
@@ -144,17 +144,17 @@ struct Cfg;
 template<>
 struct Cfg<14>
 {
-    static constexpr int D = 14, N = 15, NR = 29, NE = 16, NT = 64, TG = 8, S = 2;
+    static constexpr int D = 14, N = 15, NR = 29, NE = 16, NT = 64, TG = 8, S = 2, NNZ = 72, IMG = 512, WPE = 4;
 };
 template<>
 struct Cfg<86>
 {
-    static constexpr int D = 86, N = 87, NR = 309, NE = 157, NT = 256, TG = 16, S = 6;
+    static constexpr int D = 86, N = 87, NR = 309, NE = 157, NT = 256, TG = 16, S = 6, NNZ = 800, IMG = 4224, WPE = 2;
 };
 template<>
 struct Cfg<87>
 {
-    static constexpr int D = 87, N = 88, NR = 311, NE = 157, NT = 256, TG = 16, S = 6;
+    static constexpr int D = 87, N = 88, NR = 311, NE = 157, NT = 256, TG = 16, S = 6, NNZ = 800, IMG = 4224, WPE = 2;
 };
 
 template<int NET>
@@ -163,27 +163,41 @@ struct Lds
     using C = Cfg<NET>;
     static constexpr int D = C::D, N = C::N, NR = C::NR, NE = C::NE;
     static constexpr int NU = N * (N + 1) / 2; // packed normalised rows
-    // phase-1/2 arrays (rates and assembly), aliased with the packed rows of the LU
+    // phase arrays (rates and assembly), aliased with the packed rows of the LU
     static constexpr int oBE = 0, oMu = oBE + D, oEff = oMu + D, oDeff = oEff + NE, oL = oDeff + NE, oDl = oL + NE,
                          oRate = oDl + NE, oDrate = oRate + NR, oVal = oDrate + NR, oDval = oVal + NR,
                          oTr = oDval + NR, oJok = oTr + NR * MAXQ, oDYdot = oJok + NR, oDYdT = oDYdot + D,
-                         oCorr = oDYdT + D, oEv = oCorr + D, oEos = oEv + 12, phaseEnd = oEos + 4;
+                         oCorr = oDYdT + D, oEv = oCorr + D, oEos = oEv + 12, oJv = oEos + 4,
+                         phaseEnd = oJv + C::NNZ;
     static constexpr int unionSize = phaseEnd > NU ? phaseEnd : NU;
     static constexpr int oY = unionSize, oYb = oY + D, oX = oYb + D, oW = oX + N + 1, total = oW + 2 * C::TG * C::S;
 };
 
+// Lane of the first special-rate group task / of the electron-capture task.
+template<int NET>
+__device__ __forceinline__ int special_task(int tid)
+{
+    // net86/net87: two groups on lanes 32, 33 of wavefronts 0, 1, 2 (next to the fitted rates)
+    if (NET != 14) return (tid & 63) >= 32 && (tid & 63) < 34 && tid < 192 ? (tid >> 6) * 2 + ((tid & 63) - 32) : -1;
+    return tid >= 32 && tid < 38 ? tid - 32 : -1;
+}
+
 // ---------------------------------------------------------------------------------------------
-// The FP64 kernel: one workgroup integrates one particle over one hydro step.
+// The FP64 kernel: persistent workgroups take particles from a work queue; one workgroup
+// integrates one particle over one hydro step at a time.
 // ---------------------------------------------------------------------------------------------
 template<int NET, int EOSK>
-__global__ void __launch_bounds__(Cfg<NET>::NT) nnet_fp64_kernel(NetDev nd, EosDev ed, BatchDev bd, StepParams sp)
+__global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
+    nnet_fp64_kernel(NetDev nd, EosDev ed, BatchDev bd, StepParams sp)
 {
     using C  = Cfg<NET>;
     using L  = Lds<NET>;
     constexpr int D = C::D, N = C::N, NT = C::NT, TG = C::TG, S = C::S;
     constexpr int Dr = (NET == 14) ? 14 : 86; // species of the rate functor's BE/screening loops
 
-    __shared__ double sm[L::total];
+    __shared__ double   sm[L::total];
+    __shared__ uint32_t simg[C::IMG];
+    __shared__ unsigned s_p;
     double* sBE   = sm + L::oBE;
     double* sMu   = sm + L::oMu;
     double* sEff  = sm + L::oEff;
@@ -201,601 +215,620 @@ __global__ void __launch_bounds__(Cfg<NET>::NT) nnet_fp64_kernel(NetDev nd, EosD
     double* sCorr = sm + L::oCorr;
     double* sEv   = sm + L::oEv;
     double* sEos  = sm + L::oEos;
+    double* sJv   = sm + L::oJv;
     double* sU    = sm;
     double* sY    = sm + L::oY;
     double* sYb   = sm + L::oYb;
     double* sX    = sm + L::oX;
     double* sW    = sm + L::oW;
 
-    const int    tid = threadIdx.x;
-    const int    tr = tid / TG, tc = tid % TG;
-    const size_t p   = blockIdx.x;
-    if (p >= bd.n) return;
+    const int tid = threadIdx.x;
+    const int tr = tid / TG, tc = tid % TG;
 
-    const double rho = bd.rho[p];
-    double       T   = bd.temp[p];
-    double       dt  = bd.dt[p];
+    // stage the network's assembly image once per workgroup
+    for (int w = tid; w < nd.img_words; w += NT)
+        simg[w] = nd.img[w];
+    const uint32_t* rx     = simg + nd.oRx;
+    const uint16_t* ent16  = reinterpret_cast<const uint16_t*>(simg) + nd.oEnt16;
+    const uint16_t* spp16  = reinterpret_cast<const uint16_t*>(simg) + nd.oSpPtr16;
+    const uint32_t* spent  = simg + nd.oSpEnt;
+    const uint16_t* nzp16  = reinterpret_cast<const uint16_t*>(simg) + nd.oNzPtr16;
+    const uint32_t* nzent  = simg + nd.oNzEnt;
+    const uint16_t* rowp16 = reinterpret_cast<const uint16_t*>(simg) + nd.oRowPtr16;
+    const uint8_t*  nzcol  = reinterpret_cast<const uint8_t*>(simg) + nd.oNzCol8;
 
-    // parallel_nuclear_net_impl.cpp:83
-    if (!(rho > k::minRho && T > k::minTemp))
+    for (;;)
     {
-        if (tid == 0)
-        {
-            if (bd.status) bd.status[p] = NNET_STATUS_SKIPPED;
-            if (bd.substeps) bd.substeps[p] = 0;
-            if (bd.newton) bd.newton[p] = 0;
-        }
-        return;
-    }
-    // parallel_nuclear_net_impl.cpp:89-92 (the CPU path integrates with drho/dt = 0)
-    if (tid == 0 && sp.use_drhodt)
-        if (bd.rho_m1[p] != 0) bd.rho_m1[p] = (rho - bd.rho_m1[p]) / sp.previous_dt;
-    const double drho_dt = 0;
-
-    for (int s = tid; s < D; s += NT)
-        sY[s] = bd.Y[s][p];
-
-    const double dt_tot  = sp.hydro_dt;
-    double       elapsed = 0;
-    double       T_buf   = __builtin_nan("");
-    double       att_dt = 0, att_Tb = 0;
-    long         total_iter = 0;
-    uint32_t     n_sub = 0, n_newton = 0;
-    uint8_t      status = NNET_STATUS_OK;
-
-    for (int i = 1;; ++i)
-    {
-        if (i == 1)
-        {
-            att_dt = dt;
-            att_Tb = T_buf;
-        }
-        // prepareSystemSubstep (nuclear_net.hpp:858-879)
-        const double rho_t   = rho - drho_dt * (dt_tot - elapsed);
-        double       used_dt = dt;
-        if (dt_tot - elapsed < dt) used_dt = dt_tot - elapsed;
-        if (used_dt == 0)
-        {
-            status = NNET_STATUS_ZERO_DT;
-            break;
-        }
-        // prepareSystemNR (nuclear_net.hpp:699-712)
-        const double T_loc   = (i <= 1) ? T : T_buf;
-        const double T_theta = (1 - k::theta) * T + k::theta * T_loc;
-        for (int s = tid; s < D; s += NT)
-        {
-            const double yb = (i <= 1) ? sY[s] : sYb[s];
-            sYb[s]          = (1 - k::theta) * sY[s] + k::theta * yb;
-        }
         __syncthreads();
+        if (tid == 0) s_p = atomicAdd(bd.queue, 1u);
+        __syncthreads();
+        const size_t p = s_p;
+        if (p >= bd.n) break;
 
-        // ---- EOS ----
-        double cv = 0., dpdT = 0.;
-        if constexpr (EOSK == NNET_EOS_IDEAL)
-        {
-            const double dmy = 8.317e7 / ed.mu;
-            cv               = 1.5 * dmy;
-            dpdT             = rho_t * dmy;
-        }
-        else
+        const double rho = bd.rho[p];
+        double       T   = bd.temp[p];
+        double       dt  = bd.dt[p];
+
+        // parallel_nuclear_net_impl.cpp:83
+        if (!(rho > k::minRho && T > k::minTemp))
         {
             if (tid == 0)
             {
-                double abar = 0., zbar = 0;
-                for (int s = 0; s < ed.zdim; ++s)
-                    abar += sYb[s];
-                for (int s = 0; s < ed.zdim; ++s)
-                    zbar += sYb[s] * ed.Z[s];
-                double c0, d0;
-                helmholtz_cv_dpdt(ed.grid, ed.cells, abar, zbar, T_theta, rho_t, c0, d0);
-                sEos[0] = c0;
-                sEos[1] = d0;
+                if (bd.status) bd.status[p] = NNET_STATUS_SKIPPED;
+                if (bd.substeps) bd.substeps[p] = 0;
+                if (bd.newton) bd.newton[p] = 0;
             }
+            continue;
         }
+        // parallel_nuclear_net_impl.cpp:89-92 (the CPU path integrates with drho/dt = 0)
+        if (tid == 0 && sp.use_drhodt)
+            if (bd.rho_m1[p] != 0) bd.rho_m1[p] = (rho - bd.rho_m1[p]) / sp.previous_dt;
+        const double drho_dt = 0;
 
-        // ---- rates, phase 1 ----
+        for (int s = tid; s < D; s += NT)
+            sY[s] = bd.Y[s][p];
+
+        const double dt_tot  = sp.hydro_dt;
+        double       elapsed = 0;
+        double       T_buf   = __builtin_nan("");
+        double       att_dt = 0, att_Tb = 0;
+        long         total_iter = 0;
+        uint32_t     n_sub = 0, n_newton = 0;
+        uint8_t      status = NNET_STATUS_OK;
+
+        for (int i = 1;; ++i)
         {
-            const double Tg   = T_theta;
-            const int    SB   = nd.nfit;      // special groups
-            const int    SS   = nd.nfit + 6;  // species
-            if (tid < nd.nfit)
+            if (i == 1)
             {
-                const int     f  = tid;
-                const double* fc = nd.fit + 8 * f;
-                const double  t9 = Tg / 1.e9, t913 = pow(t9, 1. / 3.), t923 = t913 * t913, t953 = pow(t9, 5. / 3.);
-                const double  t9i = 1.e0 / t9, t9i2 = t9i * t9i, t9i13 = 1.e0 / t913, t9i23 = t9i13 * t9i13,
-                             t9i43 = t9i23 * t9i23, lt9 = log(t9);
-                const double coef = fc[1] * t9i + fc[2] * t9i13 + fc[3] * t913 + fc[4] * t9 + fc[5] * t953 + fc[6] * lt9;
-                const double dcoef =
-                    (-fc[1] * t9i2 + (-fc[2] * t9i43 + fc[3] * t9i23 + 5 * fc[5] * t923) * (1. / 3.) + fc[4] +
-                     fc[6] * t9i) *
-                    1e-9;
-                const int    e   = nd.fit_e0 + f;
-                const double eff = exp(fc[0] + coef);
-                sEff[e]          = eff;
-                sDeff[e]         = eff * dcoef;
-                // reverse rate (net14.cpp:150-190, net86.cpp:524-554)
-                const double rt9  = nd.rev_div ? Tg / 1.e9 : Tg * 1e-09;
-                const double rt9i = 1. / rt9;
-                const double val1 = 11.6045 * rt9i;
-                const double val3 = val1 * rt9i * 1e-9;
-                const int    kb   = temperature_bin(Tg);
-                const double part = nd.choose[nd.ca[f] * 24 + kb] / nd.choose[nd.cb[f] * 24 + kb];
-                double       l, dl;
-                if (nd.photo[f])
-                {
-                    const double val2 = 1.5 * log(rt9);
-                    const double val4 = 1.5e-9 * rt9i;
-                    l                 = part * exp(fc[7] + coef - val1 * nd.q[f] + val2);
-                    dl                = l * (dcoef + val3 * nd.q[f] + val4);
-                }
-                else
-                {
-                    l  = part * exp(fc[7] + coef - val1 * nd.q[f]);
-                    dl = l * (dcoef + val3 * nd.q[f]);
-                }
-                sL[e]  = l;
-                sDl[e] = dl;
+                att_dt = dt;
+                att_Tb = T_buf;
             }
-            else if (tid >= SB && tid < SB + 6)
+            // prepareSystemSubstep (nuclear_net.hpp:858-879)
+            const double rho_t   = rho - drho_dt * (dt_tot - elapsed);
+            double       used_dt = dt;
+            if (dt_tot - elapsed < dt) used_dt = dt_tot - elapsed;
+            if (used_dt == 0)
             {
-                const int g = tid - SB;
-                double    r = 0, rl = 0, dr = 0, drl = 0;
-                if (g == 0)
+                status = NNET_STATUS_ZERO_DT;
+                break;
+            }
+            // prepareSystemNR (nuclear_net.hpp:699-712)
+            const double T_loc   = (i <= 1) ? T : T_buf;
+            const double T_theta = (1 - k::theta) * T + k::theta * T_loc;
+            for (int s = tid; s < D; s += NT)
+            {
+                const double yb = (i <= 1) ? sY[s] : sYb[s];
+                sYb[s]          = (1 - k::theta) * sY[s] + k::theta * yb;
+            }
+            __syncthreads();
+
+            // ---- EOS (one lane) and rates, phase 1 (all lanes) ----
+            if constexpr (EOSK != NNET_EOS_IDEAL)
+            {
+                if (tid == NT - 1)
                 {
-                    special_triple_alpha(Tg, r, rl, dr, drl);
-                    sEff[nd.sp.r3a_e] = r, sDeff[nd.sp.r3a_e] = dr, sL[nd.sp.r3a_l] = rl, sDl[nd.sp.r3a_l] = drl;
-                }
-                else if (g == 1)
-                {
-                    special_c12c12(Tg, r, dr);
-                    sEff[nd.sp.r24] = r, sDeff[nd.sp.r24] = dr;
-                }
-                else if (g == 2)
-                {
-                    special_c12o16(Tg, r, dr);
-                    sEff[nd.sp.r1216] = r, sDeff[nd.sp.r1216] = dr;
-                }
-                else if (g == 3)
-                {
-                    special_o16o16(Tg, r, dr);
-                    sEff[nd.sp.r32] = r, sDeff[nd.sp.r32] = dr;
-                }
-                else if (g == 4)
-                {
-                    special_c12ag(Tg, r, rl, dr, drl);
-                    sEff[nd.sp.rcag_e] = r, sDeff[nd.sp.rcag_e] = dr, sL[nd.sp.rcag_l] = rl, sDl[nd.sp.rcag_l] = drl;
-                }
-                else
-                {
-                    special_o16ag(Tg, r, rl, dr, drl);
-                    sEff[nd.sp.roag_e] = r, sDeff[nd.sp.roag_e] = dr, sL[nd.sp.roag_l] = rl, sDl[nd.sp.roag_l] = drl;
+                    double abar = 0., zbar = 0;
+                    for (int s = 0; s < ed.zdim; ++s)
+                        abar += sYb[s];
+                    for (int s = 0; s < ed.zdim; ++s)
+                        zbar += sYb[s] * ed.Z[s];
+                    double c0, d0;
+                    helmholtz_cv_dpdt(ed.grid, ed.cells, abar, zbar, T_theta, rho_t, c0, d0);
+                    sEos[0] = c0;
+                    sEos[1] = d0;
                 }
             }
-            else if (tid >= SS && tid < SS + Dr)
             {
-                // corrected binding energy (net14.cpp:49-75) and screening mu/kT (:577-600)
-                const int    s          = tid - SS;
-                const double kbt        = phys::Kb * Tg;
-                const double nakbt      = phys::Na * kbt;
-                const double correction = -1.5 * nakbt;
-                const double ne         = rho_t * phys::Na / 2.;
+                const double Tg = T_theta;
+                const int    SS = nd.nfit; // species lanes follow the fitted-rate lanes
+                if (tid < nd.nfit)
                 {
-                    const double ae  = pow((3. / 4.) / (phys::pi * ne), 1. / 3.);
-                    const double gam = phys::e2 / (kbt * ae);
-                    const double gmm = gam * nd.Z53[s];
-                    const double fn  = gmm > 1 ? coulomb_strong(gmm) : coulomb_weak(gmm);
-                    double       be  = nd.BE0[s] + correction;
-                    be -= nakbt * fn;
-                    sBE[s] = be;
-                }
-                {
-                    const double kbt2 = Tg * phys::Kb;
-                    const double ae   = pow(3. / (4. * phys::pi * ne), 1. / 3.);
-                    const double gam  = phys::e2 / (kbt2 * ae);
-                    const double a1 = -.898004, b1 = .96786, c1 = .220703, d1 = -.86097, e1 = 2.520058332;
-                    const double gp  = gam * nd.Z53[s];
-                    const double sq  = sqrt(gp);
-                    const double sq2 = sqrt(sq);
-                    double       mu;
-                    if (gp <= 1)
-                        mu = -(1. / 1.7320508075688772) * gp * sq + pow(gp, 1.9885) * .29561 / 1.9885;
+                    const int     f  = tid;
+                    const double* fc = nd.fit + 8 * f;
+                    const double  t9 = Tg / 1.e9, t913 = pow(t9, 1. / 3.), t923 = t913 * t913, t953 = pow(t9, 5. / 3.);
+                    const double  t9i = 1.e0 / t9, t9i2 = t9i * t9i, t9i13 = 1.e0 / t913, t9i23 = t9i13 * t9i13,
+                                 t9i43 = t9i23 * t9i23, lt9 = log(t9);
+                    const double coef =
+                        fc[1] * t9i + fc[2] * t9i13 + fc[3] * t913 + fc[4] * t9 + fc[5] * t953 + fc[6] * lt9;
+                    const double dcoef =
+                        (-fc[1] * t9i2 + (-fc[2] * t9i43 + fc[3] * t9i23 + 5 * fc[5] * t923) * (1. / 3.) + fc[4] +
+                         fc[6] * t9i) *
+                        1e-9;
+                    const int    e   = nd.fit_e0 + f;
+                    const double eff = exp(fc[0] + coef);
+                    sEff[e]          = eff;
+                    sDeff[e]         = eff * dcoef;
+                    // reverse rate (net14.cpp:150-190, net86.cpp:524-554)
+                    const double rt9  = nd.rev_div ? Tg / 1.e9 : Tg * 1e-09;
+                    const double rt9i = 1. / rt9;
+                    const double val1 = 11.6045 * rt9i;
+                    const double val3 = val1 * rt9i * 1e-9;
+                    const int    kb   = temperature_bin(Tg);
+                    const double part = nd.choose[nd.ca[f] * 24 + kb] / nd.choose[nd.cb[f] * 24 + kb];
+                    double       l, dl;
+                    if (nd.photo[f])
+                    {
+                        const double val2 = 1.5 * log(rt9);
+                        const double val4 = 1.5e-9 * rt9i;
+                        l                 = part * exp(fc[7] + coef - val1 * nd.q[f] + val2);
+                        dl                = l * (dcoef + val3 * nd.q[f] + val4);
+                    }
                     else
-                        mu = a1 * gp + 4. * (b1 * sq2 - c1 / sq2) + d1 * log(gp) - e1;
-                    if (NET != 14 && s == 1) mu = 0; // neutron (net86.cpp:603)
-                    sMu[s] = mu;
+                    {
+                        l  = part * exp(fc[7] + coef - val1 * nd.q[f]);
+                        dl = l * (dcoef + val3 * nd.q[f]);
+                    }
+                    sL[e]  = l;
+                    sDl[e] = dl;
                 }
+                else if (tid >= SS && tid < SS + Dr)
+                {
+                    // corrected binding energy (net14.cpp:49-75) and screening mu/kT (:577-600)
+                    const int    s          = tid - SS;
+                    const double kbt        = phys::Kb * Tg;
+                    const double nakbt      = phys::Na * kbt;
+                    const double correction = -1.5 * nakbt;
+                    const double ne         = rho_t * phys::Na / 2.;
+                    {
+                        const double ae  = pow((3. / 4.) / (phys::pi * ne), 1. / 3.);
+                        const double gam = phys::e2 / (kbt * ae);
+                        const double gmm = gam * nd.Z53[s];
+                        const double fn  = gmm > 1 ? coulomb_strong(gmm) : coulomb_weak(gmm);
+                        double       be  = nd.BE0[s] + correction;
+                        be -= nakbt * fn;
+                        sBE[s] = be;
+                    }
+                    {
+                        const double kbt2 = Tg * phys::Kb;
+                        const double ae   = pow(3. / (4. * phys::pi * ne), 1. / 3.);
+                        const double gam  = phys::e2 / (kbt2 * ae);
+                        const double a1 = -.898004, b1 = .96786, c1 = .220703, d1 = -.86097, e1 = 2.520058332;
+                        const double gp  = gam * nd.Z53[s];
+                        const double sq  = sqrt(gp);
+                        const double sq2 = sqrt(sq);
+                        double       mu;
+                        if (gp <= 1)
+                            mu = -(1. / 1.7320508075688772) * gp * sq + pow(gp, 1.9885) * .29561 / 1.9885;
+                        else
+                            mu = a1 * gp + 4. * (b1 * sq2 - c1 / sq2) + d1 * log(gp) - e1;
+                        if (NET != 14 && s == 1) mu = 0; // neutron (net86.cpp:603)
+                        sMu[s] = mu;
+                    }
+                }
+                // hand-fitted rates: six groups on lanes of different wavefronts
+                const int g = special_task<NET>(tid);
+                if (g >= 0)
+                {
+                    double r = 0, rl = 0, dr = 0, drl = 0;
+                    if (g == 0)
+                    {
+                        special_triple_alpha(Tg, r, rl, dr, drl);
+                        sEff[nd.sp.r3a_e] = r, sDeff[nd.sp.r3a_e] = dr, sL[nd.sp.r3a_l] = rl, sDl[nd.sp.r3a_l] = drl;
+                    }
+                    else if (g == 1)
+                    {
+                        special_c12c12(Tg, r, dr);
+                        sEff[nd.sp.r24] = r, sDeff[nd.sp.r24] = dr;
+                    }
+                    else if (g == 2)
+                    {
+                        special_c12o16(Tg, r, dr);
+                        sEff[nd.sp.r1216] = r, sDeff[nd.sp.r1216] = dr;
+                    }
+                    else if (g == 3)
+                    {
+                        special_o16o16(Tg, r, dr);
+                        sEff[nd.sp.r32] = r, sDeff[nd.sp.r32] = dr;
+                    }
+                    else if (g == 4)
+                    {
+                        special_c12ag(Tg, r, rl, dr, drl);
+                        sEff[nd.sp.rcag_e] = r, sDeff[nd.sp.rcag_e] = dr, sL[nd.sp.rcag_l] = rl,
+                        sDl[nd.sp.rcag_l] = drl;
+                    }
+                    else
+                    {
+                        special_o16ag(Tg, r, rl, dr, drl);
+                        sEff[nd.sp.roag_e] = r, sDeff[nd.sp.roag_e] = dr, sL[nd.sp.roag_l] = rl,
+                        sDl[nd.sp.roag_l] = drl;
+                    }
+                }
+                if (NET == 87 && tid == 224)
+                    electron_interpolate(nd.elogT, nd.elogR, nd.etab, Tg, sYb[86] * rho_t, sEv);
+                if (NET != 14 && tid == 225) sEff[3] = sDeff[3] = sL[3] = sDl[3] = 0; // unused slots
             }
-            else if (NET == 87 && tid == SS + Dr)
-            {
-                electron_interpolate(nd.elogT, nd.elogR, nd.etab, Tg, sYb[86] * rho_t, sEv);
-            }
-            if (NET == 86 || NET == 87)
-                if (tid == NT - 1) sEff[3] = sDeff[3] = sL[3] = sDl[3] = 0; // unused slots
-            if (NET != 14)
-                if (tid == NT - 2) sL[0] = sL[1] = sL[2] = sDl[0] = sDl[1] = sDl[2] = 0;
-        }
-        __syncthreads();
+            __syncthreads();
 
-        // ---- rates, phase 2: swap, screening, reaction-list order ----
-        {
-            const double Tg = T_theta;
-            if (tid < nd.nE)
+            // ---- rates, phase 2: swap, screening, reaction-list order ----
             {
-                const int i  = tid;
-                double    e  = sEff[i], de = sDeff[i], l = sL[i], dl = sDl[i];
-                if (nd.swap[i])
+                const double Tg = T_theta;
+                if (tid < nd.nE)
                 {
-                    double t = e;
-                    e        = l;
-                    l        = t;
-                    t        = de;
-                    de       = dl;
-                    dl       = t;
+                    const int i = tid;
+                    double    e = sEff[i], de = sDeff[i], l = sL[i], dl = sDl[i];
+                    if (nd.swap[i])
+                    {
+                        double t = e;
+                        e        = l;
+                        l        = t;
+                        t        = de;
+                        de       = dl;
+                        dl       = t;
+                    }
+                    const int* sc = nd.scr + 4 * i;
+                    double     dm = sMu[sc[0]] + sMu[sc[1]] - sMu[sc[2]];
+                    if (sc[3] >= 0) dm += sMu[sc[3]];
+                    const double EF = exp(dm);
+                    e               = e * EF;
+                    de              = de * EF - 2. * e * dm / Tg;
+                    if (nd.scr_l[i])
+                    {
+                        l  = l * EF;
+                        dl = dl * EF - 2. * l * dm / Tg;
+                    }
+                    if (nd.push_e[i] >= 0) sRate[nd.push_e[i]] = e, sDrt[nd.push_e[i]] = de;
+                    if (nd.push_l[i] >= 0) sRate[nd.push_l[i]] = l, sDrt[nd.push_l[i]] = dl;
                 }
-                const int* sc = nd.scr + 4 * i;
-                double     dm = sMu[sc[0]] + sMu[sc[1]] - sMu[sc[2]];
-                if (sc[3] >= 0) dm += sMu[sc[3]];
-                const double EF = exp(dm);
-                e               = e * EF;
-                de              = de * EF - 2. * e * dm / Tg;
-                if (nd.scr_l[i])
+                if (NET == 87 && tid == NT - 1)
                 {
-                    l  = l * EF;
-                    dl = dl * EF - 2. * l * dm / Tg;
+                    // net87.hpp:153-174
+                    const double kbt        = phys::Kb * Tg;
+                    const double nakbt      = phys::Na * kbt;
+                    const double correction = -1.5 * nakbt;
+                    sBE[86]                 = nd.BEe + correction;
+                    sBE[0] += -sEv[3];
+                    sBE[1] += -sEv[9];
+                    const double rhoElec = sYb[86] * rho_t;
+                    sRate[309]           = rhoElec == 0 ? 0 : sEv[0] / rhoElec;
+                    sDrt[309]            = rhoElec == 0 ? 0 : sEv[1] / rhoElec;
+                    sRate[310]           = rhoElec == 0 ? 0 : sEv[6] / rhoElec;
+                    sDrt[310]            = rhoElec == 0 ? 0 : sEv[7] / rhoElec;
                 }
-                if (nd.push_e[i] >= 0) sRate[nd.push_e[i]] = e, sDrt[nd.push_e[i]] = de;
-                if (nd.push_l[i] >= 0) sRate[nd.push_l[i]] = l, sDrt[nd.push_l[i]] = dl;
             }
-            if (NET == 87 && tid == NT - 1)
-            {
-                // net87.hpp:153-174
-                const double kbt        = phys::Kb * Tg;
-                const double nakbt      = phys::Na * kbt;
-                const double correction = -1.5 * nakbt;
-                sBE[86]                 = nd.BEe + correction;
-                sBE[0] += -sEv[3];
-                sBE[1] += -sEv[9];
-                const double rhoElec = sYb[86] * rho_t;
-                sRate[309]           = rhoElec == 0 ? 0 : sEv[0] / rhoElec;
-                sDrt[309]            = rhoElec == 0 ? 0 : sEv[1] / rhoElec;
-                sRate[310]           = rhoElec == 0 ? 0 : sEv[6] / rhoElec;
-                sDrt[310]            = rhoElec == 0 ? 0 : sEv[7] / rhoElec;
-            }
-        }
-        __syncthreads();
+            __syncthreads();
 
-        // ---- per-reaction mass-action factors (nuclear_net.hpp:355-371, 404-426) ----
-        for (int r = tid; r < nd.nR; r += NT)
-        {
-            double    v = sRate[r], dv = sDrt[r], jr = sRate[r];
-            const int q0 = nd.rb[r], q1 = nd.pb[r];
-            for (int q = q0; q < q1; ++q)
+            // ---- per-reaction mass-action factors (nuclear_net.hpp:355-371, 404-426) ----
+            for (int r = tid; r < nd.nR; r += NT)
             {
-                const int    c = nd.nums[q];
-                const double x = sYb[nd.ids[q]] * rho_t;
-                if (c != 1)
+                double       v = sRate[r], dv = sDrt[r], jr = sRate[r];
+                const int    q0 = rx[r] & 0xffff, q1 = rx[r] >> 16;
+                for (int q = q0; q < q1; ++q)
                 {
-                    const double fct = factorial_count(c);
-                    v                = v / fct;
-                    dv               = dv / fct;
-                    jr               = jr / fct;
+                    const int    c = ent16[q] >> 8;
+                    const double x = sYb[ent16[q] & 0xff] * rho_t;
+                    if (c != 1)
+                    {
+                        const double fct = factorial_count(c);
+                        v                = v / fct;
+                        dv               = dv / fct;
+                        jr               = jr / fct;
+                    }
+                    v  = v * pow_count(x, c);
+                    dv = dv * pow_count(x, c);
+                    jr = jr * pow_count(x, c - 1);
                 }
-                v  = v * pow_count(x, c);
-                dv = dv * pow_count(x, c);
-                jr = jr * pow_count(x, c - 1);
+                sVal[r]       = v / rho_t;
+                sDval[r]      = dv / rho_t;
+                const bool ok = jr > k::epsilonSystem;
+                sJok[r]       = ok ? 1. : 0.;
+                if (ok)
+                    for (int qa = q0; qa < q1; ++qa)
+                    {
+                        const int b   = ent16[qa] & 0xff;
+                        double    tr_ = jr;
+                        for (int o = q0; o < q1; ++o)
+                            if ((ent16[o] & 0xff) != b) tr_ *= sYb[ent16[o] & 0xff] * rho_t;
+                        sTr[r * MAXQ + (qa - q0)] = tr_;
+                    }
             }
-            sVal[r]  = v / rho_t;
-            sDval[r] = dv / rho_t;
-            const bool ok = jr > k::epsilonSystem;
-            sJok[r]       = ok ? 1. : 0.;
-            if (ok)
-                for (int qa = q0; qa < q1; ++qa)
-                {
-                    const int b  = nd.ids[qa];
-                    double    tr_ = jr;
-                    for (int o = q0; o < q1; ++o)
-                        if (nd.ids[o] != b) tr_ *= sYb[nd.ids[o]] * rho_t;
-                    sTr[r * MAXQ + (qa - q0)] = tr_;
-                }
-        }
-        __syncthreads();
+            __syncthreads();
 
-        // Jacobian block entry J[a][b] from its ordered contribution list.
-        auto jac = [&](int a, int b) -> double
-        {
-            double    J  = 0.;
-            const int e0 = nd.jp_ptr[a * D + b], e1 = nd.jp_ptr[a * D + b + 1];
-            for (int e = e0; e < e1; ++e)
+            // ---- Jacobian non-zeros (firstOrderDYFromReactions) and species sums
+            //      (derivativesFromReactions), each summed in the reference's order ----
+            for (int e = tid; e < nd.nnz; e += NT)
             {
-                const int r = nd.j_r[e];
-                if (sJok[r] == 0.) continue;
-                const int    c = nd.j_c[e];
-                const double t = sTr[r * MAXQ + nd.j_q[e]];
-                if (c < 0) J -= t * (double)(-c);
-                else
-                    J += t * (double)c;
+                double    J  = 0.;
+                const int k0 = nzp16[e], k1 = nzp16[e + 1];
+                for (int kk = k0; kk < k1; ++kk)
+                {
+                    const uint32_t w = nzent[kk];
+                    const int      r = w & 0xfff;
+                    if (sJok[r] == 0.) continue;
+                    const int    c = (int8_t)(w >> 16);
+                    const double t = sTr[r * MAXQ + ((w >> 12) & 0xf)];
+                    if (c < 0) J -= t * (double)(-c);
+                    else
+                        J += t * (double)c;
+                }
+                sJv[e] = J;
             }
-            return J;
-        };
-
-        // ---- species sums (derivativesFromReactions) and RHS corrections ----
-        if (tid < D)
-        {
-            const int s  = tid;
-            double    a  = 0., b = 0.;
-            const int e0 = nd.sp_ptr[s], e1 = nd.sp_ptr[s + 1];
-            for (int e = e0; e < e1; ++e)
             {
-                const int r = nd.sp_r[e], c = nd.sp_c[e];
-                if (c < 0)
+                const int s = tid - (NT - D);
+                if (s >= 0)
                 {
-                    a -= sVal[r] * (double)(-c);
-                    b -= sDval[r] * (double)(-c);
-                }
-                else
-                {
-                    a += sVal[r] * (double)c;
-                    b += sDval[r] * (double)c;
+                    double    a = 0., b = 0.;
+                    const int e0 = spp16[s], e1 = spp16[s + 1];
+                    for (int e = e0; e < e1; ++e)
+                    {
+                        const uint32_t w = spent[e];
+                        const int      r = w & 0xffff, c = (int8_t)(w >> 16);
+                        if (c < 0)
+                        {
+                            a -= sVal[r] * (double)(-c);
+                            b -= sDval[r] * (double)(-c);
+                        }
+                        else
+                        {
+                            a += sVal[r] * (double)c;
+                            b += sDval[r] * (double)c;
+                        }
+                    }
+                    sDYd[s] = a;
+                    sDYT[s] = b;
                 }
             }
-            sDYd[s] = a;
-            sDYT[s] = b;
-        }
-        {
-            const int row = (NT >= 2 * D) ? tid - (NT - D) : tid;
-            if (row >= 0 && row < D)
+            __syncthreads();
+            // RHS correction sum_j J[a][j] (Yg[j] - Y[j]) over the row's non-zeros, j ascending
+            if (tid < D)
             {
                 double    corr = 0;
-                const int e0 = nd.row_ptr[row], e1 = nd.row_ptr[row + 1];
+                const int e0 = rowp16[tid], e1 = rowp16[tid + 1];
                 for (int e = e0; e < e1; ++e)
                 {
-                    const int b = nd.row_b[e];
-                    corr += jac(row, b) * (sYb[b] - sY[b]);
+                    const int b = nzcol[e];
+                    corr += sJv[e] * (sYb[b] - sY[b]);
                 }
-                sCorr[row] = corr;
+                sCorr[tid] = corr;
             }
-        }
-        __syncthreads();
+            __syncthreads();
 
-        // ---- Newton matrix into registers (nuclear_net.hpp:551-608) ----
-        double cvv = cv, dpdTv = dpdT;
-        if constexpr (EOSK != NNET_EOS_IDEAL)
-        {
-            cvv   = sEos[0];
-            dpdTv = sEos[1];
-        }
-        const double mthdt   = -k::theta * used_dt;
-        const double thdt    = k::theta * used_dt;
-        const double drho    = drho_dt * used_dt;
-        const double value_1 = dpdTv * drho / (rho_t * rho_t);
-        double       M[S][S];
-#pragma unroll
-        for (int a = 0; a < S; ++a)
-#pragma unroll
-            for (int b = 0; b < S; ++b)
+            // ---- Newton matrix into registers (nuclear_net.hpp:551-608) ----
+            double cvv, dpdTv;
+            if constexpr (EOSK == NNET_EOS_IDEAL)
             {
-                const int ri = tr + TG * a, ci = tc + TG * b;
-                double    v  = 0.;
-                if (ri < N && ci <= N)
+                const double dmy = 8.317e7 / ed.mu;
+                cvv              = 1.5 * dmy;
+                dpdTv            = rho_t * dmy;
+            }
+            else
+            {
+                cvv   = sEos[0];
+                dpdTv = sEos[1];
+            }
+            const double mthdt   = -k::theta * used_dt;
+            const double thdt    = k::theta * used_dt;
+            const double drho    = drho_dt * used_dt;
+            const double value_1 = dpdTv * drho / (rho_t * rho_t);
+            double       M[S][S];
+#pragma unroll
+            for (int a = 0; a < S; ++a)
+#pragma unroll
+                for (int b = 0; b < S; ++b)
                 {
-                    if (ri == 0)
+                    const int ri = tr + TG * a, ci = tc + TG * b;
+                    const int nzi = nd.slotmap[(a * S + b) * NT + tid];
+                    double    v   = 0.;
+                    if (ri < N && ci <= N)
                     {
-                        if (ci == 0) v = 1 - k::theta * value_1 / cvv;
-                        else if (ci < N)
-                            v = -sBE[ci - 1] / cvv;
-                        else
-                            v = T * value_1 / cvv;
-                    }
-                    else
-                    {
-                        const int s = ri - 1;
-                        if (ci == 0) v = sDYT[s] * mthdt;
-                        else if (ci < N)
+                        if (ri == 0)
                         {
-                            const double J = jac(s, ci - 1);
-                            v              = (ci == ri) ? 1. - thdt * J : mthdt * J;
+                            if (ci == 0) v = 1 - k::theta * value_1 / cvv;
+                            else if (ci < N)
+                                v = -sBE[ci - 1] / cvv;
+                            else
+                                v = T * value_1 / cvv;
                         }
                         else
                         {
-                            double rhs = sDYd[s] * used_dt;
-                            rhs += mthdt * sDYT[s] * (T_theta - T);
-                            rhs += mthdt * sCorr[s];
-                            v = rhs;
+                            const int s = ri - 1;
+                            if (ci == 0) v = sDYT[s] * mthdt;
+                            else if (ci < N)
+                            {
+                                const double J = nzi >= 0 ? sJv[nzi] : 0.;
+                                v              = (ci == ri) ? 1. - thdt * J : mthdt * J;
+                            }
+                            else
+                            {
+                                double rhs = sDYd[s] * used_dt;
+                                rhs += mthdt * sDYT[s] * (T_theta - T);
+                                rhs += mthdt * sCorr[s];
+                                v = rhs;
+                            }
                         }
                     }
+                    M[a][b] = v;
                 }
-                M[a][b] = v;
-            }
-        __syncthreads(); // the packed rows below alias the assembly arrays
+            __syncthreads(); // the packed rows below alias the assembly arrays
 
-        // ---- LU: right-looking, unpivoted, reference update order (eigen.hpp:182-206) ----
+            // ---- LU: right-looking, unpivoted, reference update order (eigen.hpp:182-206);
+            //      pivot rows are normalised with one reciprocal per row ----
 #pragma unroll
-        for (int ka = 0; ka < S; ++ka)
-        {
-            for (int kr = 0; kr < TG; ++kr)
+            for (int ka = 0; ka < S; ++ka)
             {
-                const int kk = ka * TG + kr;
-                if (kk >= N) break;
-                double*   Uk = sU + (kk * N - kk * (kk - 1) / 2); // packed row kk: columns kk+1 .. N
-                // phase A: normalise row kk (owners: tr == kr) and publish column kk
-                const double diag = __shfl(M[ka][ka], kr, TG);
-                if (tr == kr)
+                for (int kr = 0; kr < TG; ++kr)
                 {
+                    const int kk = ka * TG + kr;
+                    if (kk >= N) break;
+                    double* Uk = sU + (kk * N - kk * (kk - 1) / 2); // packed row kk: columns kk+1 .. N
+                    const double diag = __shfl(M[ka][ka], kr, TG);
+                    if (tr == kr)
+                    {
+                        const double rinv = 1. / diag;
+#pragma unroll
+                        for (int b = ka; b < S; ++b)
+                        {
+                            const int ci = tc + TG * b;
+                            if (ci > kk && ci <= N) Uk[ci - kk - 1] = M[ka][b] * rinv;
+                        }
+                    }
+                    if (tc == kr)
+                    {
+#pragma unroll
+                        for (int a = ka; a < S; ++a)
+                        {
+                            const int ri = tr + TG * a;
+                            if (ri > kk && ri < N) sW[(kk & 1) * TG * S + ri] = M[a][ka];
+                        }
+                    }
+                    __syncthreads();
+                    double u[S];
 #pragma unroll
                     for (int b = ka; b < S; ++b)
                     {
                         const int ci = tc + TG * b;
-                        if (ci > kk && ci <= N) Uk[ci - kk - 1] = M[ka][b] / diag;
+                        u[b]         = (ci > kk && ci <= N) ? Uk[ci - kk - 1] : 0.;
                     }
-                }
-                if (tc == kr)
-                {
 #pragma unroll
                     for (int a = ka; a < S; ++a)
                     {
                         const int ri = tr + TG * a;
-                        if (ri > kk && ri < N) sW[(kk & 1) * TG * S + ri] = M[a][ka];
-                    }
-                }
-                __syncthreads();
-                // phase B: rank-1 update of the trailing rows
-                double u[S];
-#pragma unroll
-                for (int b = ka; b < S; ++b)
-                {
-                    const int ci = tc + TG * b;
-                    u[b]         = (ci > kk && ci <= N) ? Uk[ci - kk - 1] : 0.;
-                }
-#pragma unroll
-                for (int a = ka; a < S; ++a)
-                {
-                    const int ri = tr + TG * a;
-                    if (ri > kk && ri < N)
-                    {
-                        const double w = sW[(kk & 1) * TG * S + ri];
-                        if (fabs(w) > k::epsilonSystem)
+                        if (ri > kk && ri < N)
                         {
-#pragma unroll
-                            for (int b = ka; b < S; ++b)
+                            const double w = sW[(kk & 1) * TG * S + ri];
+                            if (fabs(w) > k::epsilonSystem)
                             {
-                                const int ci = tc + TG * b;
-                                if (ci > kk && ci <= N) M[a][b] = M[a][b] - w * u[b];
+#pragma unroll
+                                for (int b = ka; b < S; ++b)
+                                {
+                                    const int ci = tc + TG * b;
+                                    if (ci > kk && ci <= N) M[a][b] = M[a][b] - w * u[b];
+                                }
                             }
                         }
                     }
                 }
             }
-        }
-        __syncthreads();
+            __syncthreads();
 
-        // ---- back substitution on the packed unit-upper rows (first wavefront) ----
-        if (tid < 64)
-        {
-            constexpr int RS = (N + 63) / 64;
-            double        acc[RS];
-#pragma unroll
-            for (int s = 0; s < RS; ++s)
+            // ---- back substitution on the packed unit-upper rows (first wavefront) ----
+            if (tid < 64)
             {
-                const int ri = tid + 64 * s;
-                acc[s]       = (ri < N) ? sU[ri * N - ri * (ri - 1) / 2 + (N - ri - 1)] : 0.;
-            }
-            for (int j = N - 1; j >= 0; --j)
-            {
-                const int    s  = j / 64;
-                double       xj = 0.;
+                constexpr int RS = (N + 63) / 64;
+                double        acc[RS];
 #pragma unroll
-                for (int ss = 0; ss < RS; ++ss)
-                    if (ss == s) xj = __shfl(acc[ss], j % 64, 64);
-                if (tid == 0) sX[j] = xj;
-#pragma unroll
-                for (int ss = 0; ss < RS; ++ss)
+                for (int s = 0; s < RS; ++s)
                 {
-                    const int ri = tid + 64 * ss;
-                    if (ri < j) acc[ss] = acc[ss] - sU[ri * N - ri * (ri - 1) / 2 + (j - ri - 1)] * xj;
+                    const int ri = tid + 64 * s;
+                    acc[s]       = (ri < N) ? sU[ri * N - ri * (ri - 1) / 2 + (N - ri - 1)] : 0.;
+                }
+                for (int j = N - 1; j >= 0; --j)
+                {
+                    const int s  = j / 64;
+                    double    xj = 0.;
+#pragma unroll
+                    for (int ss = 0; ss < RS; ++ss)
+                        if (ss == s) xj = __shfl(acc[ss], j % 64, 64);
+                    if (tid == 0) sX[j] = xj;
+#pragma unroll
+                    for (int ss = 0; ss < RS; ++ss)
+                    {
+                        const int ri = tid + 64 * ss;
+                        if (ri < j) acc[ss] = acc[ss] - sU[ri * N - ri * (ri - 1) / 2 + (j - ri - 1)] * xj;
+                    }
                 }
             }
-        }
-        __syncthreads();
-        ++n_newton;
-        ++total_iter;
+            __syncthreads();
+            ++n_newton;
+            ++total_iter;
 
-        // ---- finalizeSystemSubstep / finalizeSystemNR (nuclear_net.hpp:724-782, 887-922) ----
-        double u = dt;
-        if (dt_tot - elapsed < dt) u = dt_tot - elapsed;
-        double       timestep = 0;
-        bool         conv     = false;
-        const double last_T   = T_buf;
-        const double T_new    = T + sX[0];
-        int          nanflag  = 0;
-        double       yb_new   = 0.;
-        if (tid < D)
-        {
-            yb_new  = sY[tid] + sX[tid + 1];
-            nanflag = isnan(yb_new) ? 1 : 0;
-        }
-        const bool has_nan = __syncthreads_or(nanflag) || isnan(T_new);
-        if (has_nan || T_new < 0)
-        {
-            u *= k::nanDtStep;
-            i     = 0;
-            T_buf = T_new;
-            if (tid < D) sYb[tid] = yb_new;
-        }
-        else
-        {
-            const double dT_T = fabs((T_new - T) / T_new);
-            if (dT_T > k::NR_dTOverTempTarget * k::NR_dTOverTempTol)
+            // ---- finalizeSystemSubstep / finalizeSystemNR (nuclear_net.hpp:724-782, 887-922) ----
+            double u = dt;
+            if (dt_tot - elapsed < dt) u = dt_tot - elapsed;
+            double       timestep = 0;
+            bool         conv     = false;
+            const double last_T   = T_buf;
+            const double T_new    = T + sX[0];
+            int          nanflag  = 0;
+            double       yb_new   = 0.;
+            if (tid < D)
             {
-                u *= k::NR_dTOverTempTol / dT_T;
+                yb_new  = sY[tid] + sX[tid + 1];
+                nanflag = isnan(yb_new) ? 1 : 0;
+            }
+            const bool has_nan = __syncthreads_or(nanflag) || isnan(T_new);
+            if (has_nan || T_new < 0)
+            {
+                u *= k::nanDtStep;
                 i     = 0;
-                T_buf = T;
-                if (tid < D) sYb[tid] = sY[tid];
+                T_buf = T_new;
+                if (tid < D) sYb[tid] = yb_new;
             }
             else
             {
-                if (tid < D) sYb[tid] = (yb_new <= k::epsilonVector) ? 0. : yb_new;
-                T_buf                  = T_new;
-                const double correction = fabs((T_buf - last_T) / T_buf);
-                if (correction < k::NR_itTol || i >= k::NR_maxIt)
+                const double dT_T = fabs((T_new - T) / T_new);
+                if (dT_T > k::NR_dTOverTempTarget * k::NR_dTOverTempTol)
                 {
-                    timestep = u;
-                    u        = (dT_T == 0 ? k::maxDtStep : k::NR_dTOverTempTarget / dT_T) * timestep;
-                    u        = dmin(u, timestep * (double)k::maxDtStep);
-                    u        = dmax(u, timestep * (double)k::minDtStep);
-                    u        = dmin(u, (double)k::NR_maxDt);
-                    conv     = true;
+                    u *= k::NR_dTOverTempTol / dT_T;
+                    i     = 0;
+                    T_buf = T;
+                    if (tid < D) sYb[tid] = sY[tid];
+                }
+                else
+                {
+                    if (tid < D) sYb[tid] = (yb_new <= k::epsilonVector) ? 0. : yb_new;
+                    T_buf                   = T_new;
+                    const double correction = fabs((T_buf - last_T) / T_buf);
+                    if (correction < k::NR_itTol || i >= k::NR_maxIt)
+                    {
+                        timestep = u;
+                        u        = (dT_T == 0 ? k::maxDtStep : k::NR_dTOverTempTarget / dT_T) * timestep;
+                        u        = dmin(u, timestep * (double)k::maxDtStep);
+                        u        = dmax(u, timestep * (double)k::minDtStep);
+                        u        = dmin(u, (double)k::NR_maxDt);
+                        conv     = true;
+                    }
                 }
             }
-        }
-        if (dt_tot - elapsed < dt)
-        {
-            if (u < dt_tot - elapsed) dt = u;
-        }
-        else
-            dt = u;
-        __syncthreads();
-        if (conv)
-        {
-            if (tid < D) sY[tid] = sYb[tid];
-            T = T_buf;
-            i = 0;
-            elapsed += timestep;
-            ++n_sub;
-            if ((dt_tot - elapsed) / dt_tot < k::substep_dtTol)
+            if (dt_tot - elapsed < dt)
             {
-                __syncthreads();
+                if (u < dt_tot - elapsed) dt = u;
+            }
+            else
+                dt = u;
+            if (conv)
+            {
+                if (tid < D) sY[tid] = sYb[tid];
+                T = T_buf;
+                i = 0;
+                elapsed += timestep;
+                ++n_sub;
+                if ((dt_tot - elapsed) / dt_tot < k::substep_dtTol) break;
+            }
+            else if (i == 0)
+            {
+                const bool same_dt = __double_as_longlong(dt) == __double_as_longlong(att_dt);
+                const bool same_T  = __double_as_longlong(T_buf) == __double_as_longlong(att_Tb);
+                if (same_dt && same_T)
+                {
+                    status = NNET_STATUS_STALLED;
+                    break;
+                }
+            }
+            if (sp.max_iter > 0 && total_iter >= sp.max_iter)
+            {
+                status = NNET_STATUS_MAX_ITER;
                 break;
             }
-        }
-        else if (i == 0)
-        {
-            const bool same_dt = __double_as_longlong(dt) == __double_as_longlong(att_dt);
-            const bool same_T  = __double_as_longlong(T_buf) == __double_as_longlong(att_Tb);
-            if (same_dt && same_T)
-            {
-                status = NNET_STATUS_STALLED;
-                break;
-            }
-        }
-        if (sp.max_iter > 0 && total_iter >= sp.max_iter)
-        {
-            status = NNET_STATUS_MAX_ITER;
-            break;
+            __syncthreads();
         }
         __syncthreads();
+        for (int s = tid; s < D; s += NT)
+            bd.Y[s][p] = sY[s];
+        if (tid == 0)
+        {
+            bd.temp[p] = T;
+            bd.dt[p]   = dt;
+            if (bd.status) bd.status[p] = status;
+            if (bd.substeps) bd.substeps[p] = n_sub;
+            if (bd.newton) bd.newton[p] = n_newton;
+        }
     }
-    __syncthreads();
-    for (int s = tid; s < D; s += NT)
-        bd.Y[s][p] = sY[s];
-    if (tid == 0)
-    {
-        bd.temp[p] = T;
-        bd.dt[p]   = dt;
-        if (bd.status) bd.status[p] = status;
-        if (bd.substeps) bd.substeps[p] = n_sub;
-        if (bd.newton) bd.newton[p] = n_newton;
-    }
+}
+
+__global__ void nnet_queue_reset(unsigned int* q, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) q[i] = 0u;
 }
 
 } // namespace nnet_amd
@@ -814,7 +847,10 @@ struct nnet_ctx
     HelmGrid* d_grid  = nullptr;
     double*   d_cells = nullptr;
     bool      helm_ok = false;
-    // per-call scratch (grow only)
+    // work-queue counters, one per batch of a call
+    unsigned int* d_queue = nullptr;
+    int           n_queue = 0;
+    int           n_cu    = 0;
     std::vector<void*> owned;
 };
 
@@ -1010,82 +1046,158 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
     }
     SpecialIdx si = kind == 14 ? SpecialIdx{0, 0, 13, 14, 15, 1, 1, 2, 2} : SpecialIdx{4, 4, 0, 1, 2, 5, 5, 6, 6};
 
-    // ---- assembly maps ----
-    std::vector<int> sp_ptr(d + 1, 0), sp_r, sp_c;
+    // ---- assembly image (see NetDev) ----
+    // derivativesFromReactions: per species, the ordered (reaction, signed count) updates
+    std::vector<std::vector<std::pair<int, int>>> per(d);
+    for (int r = 0; r < nR; ++r)
     {
-        std::vector<std::vector<std::pair<int, int>>> per(d);
-        for (int r = 0; r < nR; ++r)
+        for (int qq = nw->rb[r]; qq < nw->pb[r]; ++qq)
+            per[nw->ids[qq]].push_back({r, -nw->nums[qq]});
+        for (int qq = nw->pb[r]; qq < nw->rb[r + 1]; ++qq)
+            per[nw->ids[qq]].push_back({r, nw->nums[qq]});
+    }
+    // firstOrderDYFromReactions: per matrix entry (a, b), the ordered (reaction, reactant
+    // position of b, signed count) updates, in the reference's loop order
+    struct Contrib
+    {
+        int r, q, c;
+    };
+    std::vector<std::vector<Contrib>> ent((size_t)d * d);
+    for (int r = 0; r < nR; ++r)
+        for (int qb = nw->rb[r]; qb < nw->pb[r]; ++qb)
         {
-            for (int qq = nw->rb[r]; qq < nw->pb[r]; ++qq)
-                per[nw->ids[qq]].push_back({r, -nw->nums[qq]});
-            for (int qq = nw->pb[r]; qq < nw->rb[r + 1]; ++qq)
-                per[nw->ids[qq]].push_back({r, nw->nums[qq]});
+            const int b = nw->ids[qb];
+            for (int o = nw->rb[r]; o < nw->pb[r]; ++o)
+                ent[(size_t)nw->ids[o] * d + b].push_back({r, qb - nw->rb[r], -nw->nums[o]});
+            for (int o = nw->pb[r]; o < nw->rb[r + 1]; ++o)
+                ent[(size_t)nw->ids[o] * d + b].push_back({r, qb - nw->rb[r], nw->nums[o]});
         }
+    std::vector<int> nz_of((size_t)d * d, -1), nz_row, nz_col;
+    for (int a = 0; a < d; ++a)
+        for (int b = 0; b < d; ++b)
+            if (!ent[(size_t)a * d + b].empty())
+            {
+                nz_of[(size_t)a * d + b] = (int)nz_row.size();
+                nz_row.push_back(a);
+                nz_col.push_back(b);
+            }
+    const int nnz = (int)nz_row.size();
+
+    std::vector<uint32_t> img;
+    auto align_word = [&]() {};
+    (void)align_word;
+    // rx
+    const int oRx = (int)img.size();
+    for (int r = 0; r < nR; ++r)
+        img.push_back((uint32_t)nw->rb[r] | ((uint32_t)nw->pb[r] << 16));
+    img.push_back((uint32_t)nw->rb[nR]);
+    // entries (halfwords)
+    const int             oEnt16 = (int)img.size() * 2;
+    std::vector<uint16_t> h;
+    for (size_t q = 0; q < nw->ids.size(); ++q)
+        h.push_back((uint16_t)(nw->ids[q] | (nw->nums[q] << 8)));
+    auto push_half = [&](std::vector<uint16_t>& v)
+    {
+        if (v.size() & 1) v.push_back(0);
+        for (size_t i = 0; i < v.size(); i += 2)
+            img.push_back((uint32_t)v[i] | ((uint32_t)v[i + 1] << 16));
+    };
+    push_half(h);
+    // species lists
+    const int oSpPtr16 = (int)img.size() * 2;
+    h.clear();
+    {
+        int acc = 0;
         for (int s = 0; s < d; ++s)
         {
-            for (auto& x : per[s])
-                sp_r.push_back(x.first), sp_c.push_back(x.second);
-            sp_ptr[s + 1] = (int)sp_r.size();
+            h.push_back((uint16_t)acc);
+            acc += (int)per[s].size();
         }
+        h.push_back((uint16_t)acc);
     }
-    std::vector<int> jp_ptr((size_t)d * d + 1, 0), j_r, j_q, j_c, row_ptr(d + 1, 0), row_b;
+    push_half(h);
+    const int oSpEnt = (int)img.size();
+    for (int s = 0; s < d; ++s)
+        for (auto& x : per[s])
+            img.push_back((uint32_t)x.first | (((uint32_t)(uint8_t)(int8_t)x.second) << 16));
+    // non-zeros
+    const int oNzPtr16 = (int)img.size() * 2;
+    h.clear();
     {
-        std::vector<std::vector<int>> er((size_t)d * d), eq((size_t)d * d), ec((size_t)d * d);
-        for (int r = 0; r < nR; ++r)
-            for (int qb = nw->rb[r]; qb < nw->pb[r]; ++qb)
-            {
-                const int b = nw->ids[qb];
-                for (int o = nw->rb[r]; o < nw->pb[r]; ++o)
-                {
-                    const size_t e = (size_t)nw->ids[o] * d + b;
-                    er[e].push_back(r), eq[e].push_back(qb - nw->rb[r]), ec[e].push_back(-nw->nums[o]);
-                }
-                for (int o = nw->pb[r]; o < nw->rb[r + 1]; ++o)
-                {
-                    const size_t e = (size_t)nw->ids[o] * d + b;
-                    er[e].push_back(r), eq[e].push_back(qb - nw->rb[r]), ec[e].push_back(nw->nums[o]);
-                }
-            }
-        for (size_t e = 0; e < (size_t)d * d; ++e)
+        int acc = 0;
+        for (int e = 0; e < nnz; ++e)
         {
-            for (size_t t = 0; t < er[e].size(); ++t)
-                j_r.push_back(er[e][t]), j_q.push_back(eq[e][t]), j_c.push_back(ec[e][t]);
-            jp_ptr[e + 1] = (int)j_r.size();
+            h.push_back((uint16_t)acc);
+            acc += (int)ent[(size_t)nz_row[e] * d + nz_col[e]].size();
         }
-        for (int a = 0; a < d; ++a)
+        h.push_back((uint16_t)acc);
+    }
+    push_half(h);
+    const int oNzEnt = (int)img.size();
+    for (int e = 0; e < nnz; ++e)
+        for (auto& x : ent[(size_t)nz_row[e] * d + nz_col[e]])
+            img.push_back((uint32_t)x.r | ((uint32_t)x.q << 12) | (((uint32_t)(uint8_t)(int8_t)x.c) << 16));
+    const int oRowPtr16 = (int)img.size() * 2;
+    h.clear();
+    {
+        int e = 0;
+        for (int a = 0; a <= d; ++a)
         {
-            for (int b = 0; b < d; ++b)
-                if (!er[(size_t)a * d + b].empty()) row_b.push_back(b);
-            row_ptr[a + 1] = (int)row_b.size();
+            while (e < nnz && nz_row[e] < a)
+                ++e;
+            h.push_back((uint16_t)e);
         }
     }
+    push_half(h);
+    const int            oNzCol8 = (int)img.size() * 4;
+    std::vector<uint8_t> c8(nz_col.begin(), nz_col.end());
+    while (c8.size() % 4)
+        c8.push_back(0);
+    for (size_t i = 0; i < c8.size(); i += 4)
+        img.push_back((uint32_t)c8[i] | ((uint32_t)c8[i + 1] << 8) | ((uint32_t)c8[i + 2] << 16) |
+                      ((uint32_t)c8[i + 3] << 24));
+
+    const int img_cap = kind == 14 ? Cfg<14>::IMG : Cfg<86>::IMG;
+    const int nnz_cap = kind == 14 ? Cfg<14>::NNZ : Cfg<86>::NNZ;
+    if ((int)img.size() > img_cap || nnz > nnz_cap || nw->rb[nR] > 65535)
+        return fail(ctx, NNET_E_NETWORK, "reaction list too large for the kernel's LDS image");
+
+    // per-lane register slots -> non-zero index ([slot][lane])
+    const int NT = kind == 14 ? Cfg<14>::NT : Cfg<86>::NT, TG = kind == 14 ? Cfg<14>::TG : Cfg<86>::TG,
+              S  = kind == 14 ? Cfg<14>::S : Cfg<86>::S;
+    std::vector<int16_t> slotmap((size_t)S * S * NT, -1);
+    for (int t = 0; t < NT; ++t)
+        for (int a = 0; a < S; ++a)
+            for (int b = 0; b < S; ++b)
+            {
+                const int ri = t / TG + TG * a, ci = t % TG + TG * b;
+                if (ri >= 1 && ri <= d && ci >= 1 && ci <= d) slotmap[(size_t)(a * S + b) * NT + t] = (int16_t)nz_of[(size_t)(ri - 1) * d + (ci - 1)];
+            }
 
     // ---- upload ----
     Blob bl;
-    size_t oBE = bl.add(BE0), oZ = bl.add(Zr), oZ53 = bl.add(Z53), oFit = bl.add(fit), oQ = bl.add(q),
-           oCh = bl.add(choose), oCa = bl.add(ca), oCb = bl.add(cb), oPh = bl.add(photo), oScr = bl.add(scr),
-           oSw = bl.add(swp), oScl = bl.add(scl), oPe = bl.add(pe), oPl = bl.add(pl),
-           oElT = bl.add(NNET_ELEC_LOGT, NNET_ELEC_NT), oElR = bl.add(NNET_ELEC_LOGRHO, NNET_ELEC_NR),
-           oElTab = bl.add(NNET_ELEC_RATE, (size_t)NNET_ELEC_NT * NNET_ELEC_NR * NNET_ELEC_NC), oRb = bl.add(nw->rb),
-           oPb = bl.add(nw->pb), oIds = bl.add(nw->ids), oNums = bl.add(nw->nums), oSpP = bl.add(sp_ptr),
-           oSpR = bl.add(sp_r), oSpC = bl.add(sp_c), oJpP = bl.add(jp_ptr), oJR = bl.add(j_r), oJQ = bl.add(j_q),
-           oJC = bl.add(j_c), oRowP = bl.add(row_ptr), oRowB = bl.add(row_b);
+    size_t oBE = bl.add(BE0), oZ53 = bl.add(Z53), oFit = bl.add(fit), oQ = bl.add(q), oCh = bl.add(choose),
+           oCa = bl.add(ca), oCb = bl.add(cb), oPh = bl.add(photo), oScr = bl.add(scr), oSw = bl.add(swp),
+           oScl = bl.add(scl), oPe = bl.add(pe), oPl = bl.add(pl), oElT = bl.add(NNET_ELEC_LOGT, NNET_ELEC_NT),
+           oElR = bl.add(NNET_ELEC_LOGRHO, NNET_ELEC_NR),
+           oElTab = bl.add(NNET_ELEC_RATE, (size_t)NNET_ELEC_NT * NNET_ELEC_NR * NNET_ELEC_NC), oImg = bl.add(img),
+           oSlot = bl.add(slotmap);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipMalloc(&nw->blob, bl.b.size()));
     HIP_TRY(ctx, hipMemcpy(nw->blob, bl.b.data(), bl.b.size(), hipMemcpyHostToDevice));
     char*   B = (char*)nw->blob;
     NetDev& n = nw->dev;
     n.net = kind, n.d = d, n.nR = nR, n.nE = nE, n.nfit = nfit, n.fit_e0 = e0, n.rev_div = kind == 14 ? 1 : 0;
-    n.sp = si;
-    n.BE0 = (double*)(B + oBE), n.Zr = (double*)(B + oZ), n.Z53 = (double*)(B + oZ53), n.fit = (double*)(B + oFit);
-    n.q = (double*)(B + oQ), n.choose = (double*)(B + oCh), n.ca = (int*)(B + oCa), n.cb = (int*)(B + oCb);
-    n.photo = (int*)(B + oPh), n.scr = (int*)(B + oScr), n.swap = (int*)(B + oSw), n.scr_l = (int*)(B + oScl);
-    n.push_e = (int*)(B + oPe), n.push_l = (int*)(B + oPl), n.elogT = (double*)(B + oElT);
-    n.elogR = (double*)(B + oElR), n.etab = (double*)(B + oElTab), n.BEe = NNET87_ELECTRON_BE_MEV[0] * NNET_MEV_TO_ERG;
-    n.rb = (int*)(B + oRb), n.pb = (int*)(B + oPb), n.ids = (int*)(B + oIds), n.nums = (int*)(B + oNums);
-    n.sp_ptr = (int*)(B + oSpP), n.sp_r = (int*)(B + oSpR), n.sp_c = (int*)(B + oSpC);
-    n.jp_ptr = (int*)(B + oJpP), n.j_r = (int*)(B + oJR), n.j_q = (int*)(B + oJQ), n.j_c = (int*)(B + oJC);
-    n.row_ptr = (int*)(B + oRowP), n.row_b = (int*)(B + oRowB);
+    n.nnz = nnz;
+    n.sp  = si;
+    n.BE0 = (double*)(B + oBE), n.Z53 = (double*)(B + oZ53), n.fit = (double*)(B + oFit), n.q = (double*)(B + oQ);
+    n.choose = (double*)(B + oCh), n.ca = (int*)(B + oCa), n.cb = (int*)(B + oCb), n.photo = (int*)(B + oPh);
+    n.scr = (int*)(B + oScr), n.swap = (int*)(B + oSw), n.scr_l = (int*)(B + oScl), n.push_e = (int*)(B + oPe);
+    n.push_l = (int*)(B + oPl), n.elogT = (double*)(B + oElT), n.elogR = (double*)(B + oElR);
+    n.etab = (double*)(B + oElTab), n.BEe = NNET87_ELECTRON_BE_MEV[0] * NNET_MEV_TO_ERG;
+    n.img = (uint32_t*)(B + oImg), n.img_words = (int)img.size(), n.oRx = oRx, n.oEnt16 = oEnt16;
+    n.oSpPtr16 = oSpPtr16, n.oSpEnt = oSpEnt, n.oNzPtr16 = oNzPtr16, n.oNzEnt = oNzEnt, n.oRowPtr16 = oRowPtr16;
+    n.oNzCol8 = oNzCol8, n.slotmap = (int16_t*)(B + oSlot);
     return NNET_OK;
 }
 
@@ -1103,7 +1215,8 @@ extern "C"
             delete c;
             return NNET_E_HIP;
         }
-        if (hipSetDevice(device) != hipSuccess)
+        if (hipSetDevice(device) != hipSuccess ||
+            hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         {
             delete c;
             return NNET_E_HIP;
@@ -1119,6 +1232,7 @@ extern "C"
         (void)hipSetDevice(c->device);
         if (c->d_grid) (void)hipFree(c->d_grid);
         if (c->d_cells) (void)hipFree(c->d_cells);
+        if (c->d_queue) (void)hipFree(c->d_queue);
         for (void* p : c->owned)
             (void)hipFree(p);
         delete c;
@@ -1248,6 +1362,15 @@ extern "C"
         HIP_TRY(c, hipSetDevice(c->device));
         hipStream_t st = (hipStream_t)stream;
         StepParams  sp{hydro_dt, previous_dt, use_drhodt, c->max_iter};
+        if (nb > c->n_queue)
+        {
+            // grows only in calls with more batches than before (a capture-safe caller keeps nb fixed)
+            HIP_TRY(c, hipStreamSynchronize(st));
+            if (c->d_queue) HIP_TRY(c, hipFree(c->d_queue));
+            c->d_queue = nullptr;
+            HIP_TRY(c, hipMalloc(&c->d_queue, sizeof(unsigned int) * std::max(nb, 8)));
+            c->n_queue = std::max(nb, 8);
+        }
         for (int bi = 0; bi < nb; ++bi)
         {
             const nnet_batch& b = batches[bi];
@@ -1284,32 +1407,35 @@ extern "C"
             for (int z = 0; z < d; ++z)
                 bd.Y[z] = b.Y[z];
             const NetDev& nd = b.network->dev;
-            if (b.n > 0x7fffffffULL) return fail(c, NNET_E_ARG, "batch too large for one launch");
-            const dim3 grid((unsigned)b.n);
-            const int  kind = b.network->kind;
+            bd.queue         = c->d_queue + bi;
+            hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, st, bd.queue, 1);
+            const int kind = b.network->kind;
+            auto      launch = [&](auto kern, int nt) -> int
+            {
+                int per_cu = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, 0) != hipSuccess || per_cu < 1)
+                    per_cu = 1;
+                const size_t wgs = std::min<size_t>(b.n, (size_t)per_cu * c->n_cu);
+                hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(nt), 0, st, nd, ed, bd, sp);
+                return 0;
+            };
             if (kind == 14)
             {
-                if (b.eos == NNET_EOS_IDEAL)
-                    hipLaunchKernelGGL((nnet_fp64_kernel<14, NNET_EOS_IDEAL>), grid, dim3(Cfg<14>::NT), 0, st, nd, ed, bd, sp);
+                if (b.eos == NNET_EOS_IDEAL) launch(nnet_fp64_kernel<14, NNET_EOS_IDEAL>, Cfg<14>::NT);
                 else
-                    hipLaunchKernelGGL((nnet_fp64_kernel<14, NNET_EOS_HELMHOLTZ>), grid, dim3(Cfg<14>::NT), 0, st, nd, ed, bd,
-                                       sp);
+                    launch(nnet_fp64_kernel<14, NNET_EOS_HELMHOLTZ>, Cfg<14>::NT);
             }
             else if (kind == 86)
             {
-                if (b.eos == NNET_EOS_IDEAL)
-                    hipLaunchKernelGGL((nnet_fp64_kernel<86, NNET_EOS_IDEAL>), grid, dim3(Cfg<86>::NT), 0, st, nd, ed, bd, sp);
+                if (b.eos == NNET_EOS_IDEAL) launch(nnet_fp64_kernel<86, NNET_EOS_IDEAL>, Cfg<86>::NT);
                 else
-                    hipLaunchKernelGGL((nnet_fp64_kernel<86, NNET_EOS_HELMHOLTZ>), grid, dim3(Cfg<86>::NT), 0, st, nd, ed, bd,
-                                       sp);
+                    launch(nnet_fp64_kernel<86, NNET_EOS_HELMHOLTZ>, Cfg<86>::NT);
             }
             else
             {
-                if (b.eos == NNET_EOS_IDEAL)
-                    hipLaunchKernelGGL((nnet_fp64_kernel<87, NNET_EOS_IDEAL>), grid, dim3(Cfg<87>::NT), 0, st, nd, ed, bd, sp);
+                if (b.eos == NNET_EOS_IDEAL) launch(nnet_fp64_kernel<87, NNET_EOS_IDEAL>, Cfg<87>::NT);
                 else
-                    hipLaunchKernelGGL((nnet_fp64_kernel<87, NNET_EOS_HELMHOLTZ>), grid, dim3(Cfg<87>::NT), 0, st, nd, ed, bd,
-                                       sp);
+                    launch(nnet_fp64_kernel<87, NNET_EOS_HELMHOLTZ>, Cfg<87>::NT);
             }
             HIP_TRY(c, hipGetLastError());
         }
