@@ -144,17 +144,17 @@ struct Cfg;
 template<>
 struct Cfg<14>
 {
-    static constexpr int D = 14, N = 15, NR = 29, NE = 16, NT = 64, TG = 8, S = 2, NNZ = 72, IMG = 512, WPE = 4;
+    static constexpr int D = 14, N = 15, NR = 29, NE = 16, NT = 64, TT = 1, WR = 1, NNZ = 72, IMG = 512, WPE = 4;
 };
 template<>
 struct Cfg<86>
 {
-    static constexpr int D = 86, N = 87, NR = 309, NE = 157, NT = 256, TG = 16, S = 6, NNZ = 800, IMG = 4224, WPE = 2;
+    static constexpr int D = 86, N = 87, NR = 309, NE = 157, NT = 256, TT = 6, WR = 2, NNZ = 800, IMG = 4224, WPE = 2;
 };
 template<>
 struct Cfg<87>
 {
-    static constexpr int D = 87, N = 88, NR = 311, NE = 157, NT = 256, TG = 16, S = 6, NNZ = 800, IMG = 4224, WPE = 2;
+    static constexpr int D = 87, N = 88, NR = 311, NE = 157, NT = 256, TT = 6, WR = 2, NNZ = 800, IMG = 4224, WPE = 2;
 };
 
 template<int NET>
@@ -170,7 +170,186 @@ struct Lds
                          oCorr = oDYdT + D, oEv = oCorr + D, oEos = oEv + 12, oJv = oEos + 4,
                          phaseEnd = oJv + C::NNZ;
     static constexpr int unionSize = phaseEnd > NU ? phaseEnd : NU;
-    static constexpr int oY = unionSize, oYb = oY + D, oX = oYb + D, oW = oX + N + 1, total = oW + 2 * C::TG * C::S;
+    static constexpr int oY = unionSize, oYb = oY + D, oX = oYb + D;
+    // LU panel buffers (two generations of: panel columns [row][4], pivot rows [column][4])
+    static constexpr int NPAD = 16 * C::TT, oPan = ((oX + N + 1) + 3) & ~3, total = oPan + 16 * NPAD;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Tile LU.  The augmented (N x N+1) Newton matrix lives in registers as 16x16 FP64 tiles in the
+// result layout of v_mfma_f64_16x16x4f64: register v of lane l of a tile holds row 16*I + (l>>4)
+// + 4v, column 16*J + (l&15).  Wave (wr, wc) of the WR x WR wave grid owns tiles (wr + WR*ti,
+// wc + WR*tj).  Elimination proceeds in blocks of four pivots (eigen.hpp:182-206 order per
+// element): the owners publish the block's four columns and four rows to LDS, every wave factors
+// the 4x4 diagonal block redundantly, derives its multipliers (A operand, rows) and normalised
+// pivot-row entries (B operand, columns) with the reference's 1e-40 skip test, and applies the
+// rank-4 trailing update to its tiles with one MFMA per tile.  One barrier per block.
+// ---------------------------------------------------------------------------------------------
+typedef double dx4 __attribute__((ext_vector_type(4)));
+
+template<int NET>
+struct TileLU
+{
+    using C = Cfg<NET>;
+    static constexpr int N = C::N, TT = C::TT, WR = C::WR, TI = TT / WR, NPAD = 16 * TT;
+    static constexpr int NS = (N + 3) / 4; // blocks of four pivots
+    static_assert(TT % WR == 0 && WR * WR * 64 == C::NT && 16 * TT >= N + 1, "tile grid");
+
+    // first packed entry of normalised row k (columns k+1 .. N)
+    static __device__ __forceinline__ int urow(int k) { return k * N - k * (k - 1) / 2 - k - 1; }
+
+    static __device__ __forceinline__ double pick4(const double (&x)[4], int q)
+    {
+        return q == 0 ? x[0] : q == 1 ? x[1] : q == 2 ? x[2] : x[3];
+    }
+
+    // the owners of block STEP's columns and rows write them to the panel buffers
+    template<int STEP>
+    static __device__ __forceinline__ void publish(const dx4 (&Mt)[TI][TI], double* pan, int wr, int wc, int lr, int lc)
+    {
+        constexpr int p0 = 4 * STEP, Tp = p0 >> 4, c0 = p0 & 15, v0 = c0 >> 2;
+        double*       P  = pan + (STEP & 1) * 8 * NPAD; // [row][q]: columns p0 .. p0+3
+        double*       RT = P + 4 * NPAD;                // [column][q]: rows p0 .. p0+3
+        if (wc == Tp % WR && lc >= c0 && lc < c0 + 4)
+        {
+            constexpr int tj = Tp / WR;
+#pragma unroll
+            for (int ti = 0; ti < TI; ++ti)
+                if (wr + WR * ti >= Tp)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        P[(16 * (wr + WR * ti) + lr + 4 * v) * 4 + (lc - c0)] = Mt[ti][tj][v];
+        }
+        if (wr == Tp % WR)
+        {
+            constexpr int ti = Tp / WR;
+#pragma unroll
+            for (int tj = 0; tj < TI; ++tj)
+                if (wc + WR * tj >= Tp) RT[(16 * (wc + WR * tj) + lc) * 4 + lr] = Mt[ti][tj][v0];
+        }
+    }
+
+    template<int STEP>
+    static __device__ __forceinline__ void step(dx4 (&Mt)[TI][TI], double* pan, double* sU, int wr, int wc, int lr,
+                                                int lc, int tid)
+    {
+        constexpr int p0 = 4 * STEP, Tp = p0 >> 4, nq = (N - p0) < 4 ? (N - p0) : 4;
+        const double* P  = pan + (STEP & 1) * 8 * NPAD;
+        const double* RT = P + 4 * NPAD;
+
+        // 4x4 diagonal block (columns p0 .. p0+3 may include the right-hand side in the last block)
+        double Dg[4][4], Un[4][4], Wt[4][4], rinv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+                Dg[r][c] = P[(p0 + r) * 4 + c];
+                Un[r][c] = 0.;
+                Wt[r][c] = 0.;
+            }
+#pragma unroll
+        for (int q = 0; q < nq; ++q)
+        {
+            rinv[q] = 1. / Dg[q][q];
+#pragma unroll
+            for (int c = q + 1; c < 4; ++c)
+                Un[q][c] = Dg[q][c] * rinv[q];
+#pragma unroll
+            for (int r = q + 1; r < nq; ++r)
+            {
+                const double w = Dg[r][q];
+                Wt[r][q]       = w;
+                if (fabs(w) > k::epsilonSystem)
+#pragma unroll
+                    for (int c = q + 1; c < 4; ++c)
+                        Dg[r][c] = fma(-w, Un[q][c], Dg[r][c]);
+            }
+        }
+        if (tid == 0)
+#pragma unroll
+            for (int q = 0; q < nq; ++q)
+#pragma unroll
+                for (int c = q + 1; c < 4; ++c)
+                    if (p0 + c <= N) sU[urow(p0 + q) + p0 + c] = Un[q][c];
+
+        // multipliers of this lane's rows (A operand: row l&15 of the tile, pivot l>>4), negated
+        double Aop[TI];
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti)
+        {
+            const int I = wr + WR * ti, i = 16 * I + lc;
+            Aop[ti]     = 0.;
+            if (I >= Tp && i >= p0 + 4 && i < N)
+            {
+                double pr[4], Lq[4] = {0., 0., 0., 0.};
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    pr[c] = P[i * 4 + c];
+#pragma unroll
+                for (int q = 0; q < nq; ++q)
+                {
+                    const double w = pr[q];
+                    if (fabs(w) > k::epsilonSystem)
+                    {
+                        Lq[q] = w;
+#pragma unroll
+                        for (int c = q + 1; c < nq; ++c)
+                            pr[c] = fma(-w, Un[q][c], pr[c]);
+                    }
+                }
+                Aop[ti] = -pick4(Lq, lr);
+            }
+        }
+        // normalised pivot-row entries of this lane's columns (B operand: pivot l>>4, column l&15)
+        double Bop[TI];
+#pragma unroll
+        for (int tj = 0; tj < TI; ++tj)
+        {
+            const int J = wc + WR * tj, j = 16 * J + lc;
+            Bop[tj]     = 0.;
+            if (J >= Tp && j >= p0 + 4 && j <= N)
+            {
+                double u[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    u[q] = RT[j * 4 + q];
+#pragma unroll
+                for (int q = 0; q < nq; ++q)
+                {
+                    u[q] *= rinv[q];
+#pragma unroll
+                    for (int r = q + 1; r < nq; ++r)
+                        if (fabs(Wt[r][q]) > k::epsilonSystem) u[r] = fma(-Wt[r][q], u[q], u[r]);
+                }
+                const double mine = lr < nq ? pick4(u, lr) : 0.;
+                Bop[tj]           = mine;
+                if (wr == 0 && lr < nq) sU[urow(p0 + lr) + j] = mine;
+            }
+        }
+        // rank-4 trailing update, one MFMA per tile that holds trailing rows and columns
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < TI; ++tj)
+            {
+                const int I = wr + WR * ti, J = wc + WR * tj;
+                if (I >= Tp && J >= Tp && 16 * I + 15 >= p0 + 4 && 16 * J + 15 >= p0 + 4)
+                    Mt[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(Aop[ti], Bop[tj], Mt[ti][tj], 0, 0, 0);
+            }
+        if constexpr (STEP + 1 < NS) publish<STEP + 1>(Mt, pan, wr, wc, lr, lc);
+        __syncthreads();
+        if constexpr (STEP + 1 < NS) step<STEP + 1>(Mt, pan, sU, wr, wc, lr, lc, tid);
+    }
+
+    // forward elimination of the augmented matrix; leaves the normalised rows in sU
+    static __device__ __forceinline__ void factor(dx4 (&Mt)[TI][TI], double* pan, double* sU, int wr, int wc, int lr,
+                                                  int lc, int tid)
+    {
+        publish<0>(Mt, pan, wr, wc, lr, lc);
+        __syncthreads();
+        step<0>(Mt, pan, sU, wr, wc, lr, lc, tid);
+    }
 };
 
 // Lane of the first special-rate group task / of the electron-capture task.
@@ -192,10 +371,10 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
 {
     using C  = Cfg<NET>;
     using L  = Lds<NET>;
-    constexpr int D = C::D, N = C::N, NT = C::NT, TG = C::TG, S = C::S;
+    constexpr int D = C::D, N = C::N, NT = C::NT, WR = C::WR, TI = C::TT / C::WR;
     constexpr int Dr = (NET == 14) ? 14 : 86; // species of the rate functor's BE/screening loops
 
-    __shared__ double   sm[L::total];
+    __shared__ __align__(16) double sm[L::total];
     __shared__ uint32_t simg[C::IMG];
     __shared__ unsigned s_p;
     double* sBE   = sm + L::oBE;
@@ -220,10 +399,10 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
     double* sY    = sm + L::oY;
     double* sYb   = sm + L::oYb;
     double* sX    = sm + L::oX;
-    double* sW    = sm + L::oW;
+    double* sPan  = sm + L::oPan;
 
     const int tid = threadIdx.x;
-    const int tr = tid / TG, tc = tid % TG;
+    const int lr = (tid & 63) >> 4, lc = tid & 15, wr = (tid >> 6) / WR, wc = (tid >> 6) % WR;
 
     // stage the network's assembly image once per workgroup
     for (int w = tid; w < nd.img_words; w += NT)
@@ -599,135 +778,87 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             const double thdt    = k::theta * used_dt;
             const double drho    = drho_dt * used_dt;
             const double value_1 = dpdTv * drho / (rho_t * rho_t);
-            double       M[S][S];
+            dx4 Mt[TI][TI];
 #pragma unroll
-            for (int a = 0; a < S; ++a)
+            for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-                for (int b = 0; b < S; ++b)
-                {
-                    const int ri = tr + TG * a, ci = tc + TG * b;
-                    const int nzi = nd.slotmap[(a * S + b) * NT + tid];
-                    double    v   = 0.;
-                    if (ri < N && ci <= N)
+                for (int tj = 0; tj < TI; ++tj)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
                     {
-                        if (ri == 0)
+                        const int ri = 16 * (wr + WR * ti) + lr + 4 * v, ci = 16 * (wc + WR * tj) + lc;
+                        const int nzi = nd.slotmap[((ti * TI + tj) * 4 + v) * NT + tid];
+                        double    x   = 0.;
+                        if (ri < N && ci <= N)
                         {
-                            if (ci == 0) v = 1 - k::theta * value_1 / cvv;
-                            else if (ci < N)
-                                v = -sBE[ci - 1] / cvv;
-                            else
-                                v = T * value_1 / cvv;
-                        }
-                        else
-                        {
-                            const int s = ri - 1;
-                            if (ci == 0) v = sDYT[s] * mthdt;
-                            else if (ci < N)
+                            if (ri == 0)
                             {
-                                const double J = nzi >= 0 ? sJv[nzi] : 0.;
-                                v              = (ci == ri) ? 1. - thdt * J : mthdt * J;
+                                if (ci == 0) x = 1 - k::theta * value_1 / cvv;
+                                else if (ci < N)
+                                    x = -sBE[ci - 1] / cvv;
+                                else
+                                    x = T * value_1 / cvv;
                             }
                             else
                             {
-                                double rhs = sDYd[s] * used_dt;
-                                rhs += mthdt * sDYT[s] * (T_theta - T);
-                                rhs += mthdt * sCorr[s];
-                                v = rhs;
-                            }
-                        }
-                    }
-                    M[a][b] = v;
-                }
-            __syncthreads(); // the packed rows below alias the assembly arrays
-
-            // ---- LU: right-looking, unpivoted, reference update order (eigen.hpp:182-206);
-            //      pivot rows are normalised with one reciprocal per row ----
-#pragma unroll
-            for (int ka = 0; ka < S; ++ka)
-            {
-                for (int kr = 0; kr < TG; ++kr)
-                {
-                    const int kk = ka * TG + kr;
-                    if (kk >= N) break;
-                    double* Uk = sU + (kk * N - kk * (kk - 1) / 2); // packed row kk: columns kk+1 .. N
-                    const double diag = __shfl(M[ka][ka], kr, TG);
-                    if (tr == kr)
-                    {
-                        const double rinv = 1. / diag;
-#pragma unroll
-                        for (int b = ka; b < S; ++b)
-                        {
-                            const int ci = tc + TG * b;
-                            if (ci > kk && ci <= N) Uk[ci - kk - 1] = M[ka][b] * rinv;
-                        }
-                    }
-                    if (tc == kr)
-                    {
-#pragma unroll
-                        for (int a = ka; a < S; ++a)
-                        {
-                            const int ri = tr + TG * a;
-                            if (ri > kk && ri < N) sW[(kk & 1) * TG * S + ri] = M[a][ka];
-                        }
-                    }
-                    __syncthreads();
-                    double u[S];
-#pragma unroll
-                    for (int b = ka; b < S; ++b)
-                    {
-                        const int ci = tc + TG * b;
-                        u[b]         = (ci > kk && ci <= N) ? Uk[ci - kk - 1] : 0.;
-                    }
-#pragma unroll
-                    for (int a = ka; a < S; ++a)
-                    {
-                        const int ri = tr + TG * a;
-                        if (ri > kk && ri < N)
-                        {
-                            const double w = sW[(kk & 1) * TG * S + ri];
-                            if (fabs(w) > k::epsilonSystem)
-                            {
-#pragma unroll
-                                for (int b = ka; b < S; ++b)
+                                const int s = ri - 1;
+                                if (ci == 0) x = sDYT[s] * mthdt;
+                                else if (ci < N)
                                 {
-                                    const int ci = tc + TG * b;
-                                    if (ci > kk && ci <= N) M[a][b] = M[a][b] - w * u[b];
+                                    const double J = nzi >= 0 ? sJv[nzi] : 0.;
+                                    x              = (ci == ri) ? 1. - thdt * J : mthdt * J;
+                                }
+                                else
+                                {
+                                    double rhs = sDYd[s] * used_dt;
+                                    rhs += mthdt * sDYT[s] * (T_theta - T);
+                                    rhs += mthdt * sCorr[s];
+                                    x = rhs;
                                 }
                             }
                         }
+                        Mt[ti][tj][v] = x;
                     }
-                }
-            }
-            __syncthreads();
+            __syncthreads(); // the packed rows below alias the assembly arrays
 
-            // ---- back substitution on the packed unit-upper rows (first wavefront) ----
-            if (tid < 64)
+            // ---- LU (eigen.hpp:179-206), four pivots per block ----
+            TileLU<NET>::factor(Mt, sPan, sU, wr, wc, lr, lc, tid);
+
+            // ---- back substitution on the packed unit-upper rows (eigen.hpp:208-221), 16-row
+            //      blocks from the bottom: one wavefront solves the block, all lanes update the
+            //      rows above ----
             {
-                constexpr int RS = (N + 63) / 64;
-                double        acc[RS];
-#pragma unroll
-                for (int s = 0; s < RS; ++s)
+                constexpr int BS = 16;
+                double*       sAcc = sPan; // the panel buffers are free now
+                for (int r = tid; r < N; r += NT)
+                    sAcc[r] = sU[TileLU<NET>::urow(r) + N];
+                __syncthreads();
+                for (int r1 = N; r1 > 0; r1 -= BS)
                 {
-                    const int ri = tid + 64 * s;
-                    acc[s]       = (ri < N) ? sU[ri * N - ri * (ri - 1) / 2 + (N - ri - 1)] : 0.;
-                }
-                for (int j = N - 1; j >= 0; --j)
-                {
-                    const int s  = j / 64;
-                    double    xj = 0.;
-#pragma unroll
-                    for (int ss = 0; ss < RS; ++ss)
-                        if (ss == s) xj = __shfl(acc[ss], j % 64, 64);
-                    if (tid == 0) sX[j] = xj;
-#pragma unroll
-                    for (int ss = 0; ss < RS; ++ss)
+                    const int r0 = r1 - BS > 0 ? r1 - BS : 0;
+                    if (tid < 64)
                     {
-                        const int ri = tid + 64 * ss;
-                        if (ri < j) acc[ss] = acc[ss] - sU[ri * N - ri * (ri - 1) / 2 + (j - ri - 1)] * xj;
+                        const int row = r0 + tid;
+                        double    acc = (row < r1) ? sAcc[row] : 0.;
+                        for (int j = r1 - 1; j >= r0; --j)
+                        {
+                            const double xj = __shfl(acc, j - r0, 64);
+                            if (row < j) acc = fma(-sU[TileLU<NET>::urow(row) + j], xj, acc);
+                            if (tid == j - r0) sX[j] = xj;
+                        }
                     }
+                    __syncthreads();
+                    for (int row = tid; row < r0; row += NT)
+                    {
+                        double        acc = sAcc[row];
+                        const double* Ur  = sU + TileLU<NET>::urow(row);
+                        for (int j = r1 - 1; j >= r0; --j)
+                            acc = fma(-Ur[j], sX[j], acc);
+                        sAcc[row] = acc;
+                    }
+                    __syncthreads();
                 }
             }
-            __syncthreads();
             ++n_newton;
             ++total_iter;
 
@@ -1163,16 +1294,19 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
         return fail(ctx, NNET_E_NETWORK, "reaction list too large for the kernel's LDS image");
 
     // per-lane register slots -> non-zero index ([slot][lane])
-    const int NT = kind == 14 ? Cfg<14>::NT : Cfg<86>::NT, TG = kind == 14 ? Cfg<14>::TG : Cfg<86>::TG,
-              S  = kind == 14 ? Cfg<14>::S : Cfg<86>::S;
-    std::vector<int16_t> slotmap((size_t)S * S * NT, -1);
+    const int NT = kind == 14 ? Cfg<14>::NT : Cfg<86>::NT, TT = kind == 14 ? Cfg<14>::TT : Cfg<86>::TT,
+              WR = kind == 14 ? Cfg<14>::WR : Cfg<86>::WR, TI = TT / WR;
+    std::vector<int16_t> slotmap((size_t)TI * TI * 4 * NT, -1);
     for (int t = 0; t < NT; ++t)
-        for (int a = 0; a < S; ++a)
-            for (int b = 0; b < S; ++b)
-            {
-                const int ri = t / TG + TG * a, ci = t % TG + TG * b;
-                if (ri >= 1 && ri <= d && ci >= 1 && ci <= d) slotmap[(size_t)(a * S + b) * NT + t] = (int16_t)nz_of[(size_t)(ri - 1) * d + (ci - 1)];
-            }
+        for (int ti = 0; ti < TI; ++ti)
+            for (int tj = 0; tj < TI; ++tj)
+                for (int v = 0; v < 4; ++v)
+                {
+                    const int l = t & 63, wr = (t >> 6) / WR, wc = (t >> 6) % WR;
+                    const int ri = 16 * (wr + WR * ti) + (l >> 4) + 4 * v, ci = 16 * (wc + WR * tj) + (l & 15);
+                    if (ri >= 1 && ri <= d && ci >= 1 && ci <= d)
+                        slotmap[(size_t)((ti * TI + tj) * 4 + v) * NT + t] = (int16_t)nz_of[(size_t)(ri - 1) * d + (ci - 1)];
+                }
 
     // ---- upload ----
     Blob bl;
