@@ -62,7 +62,7 @@ typedef struct nnet_network nnet_network;
 /* Context on one HIP device.  Replaces the reference's per-call cudaMalloc/cudaMemcpy of
  * tables and functors (helmholtz.cpp:174-218, net87.cpp:89-99): constant tables are uploaded
  * once here.  helm_table_path may be NULL (ideal gas only) or the path of a binary Helmholtz
- * table (see nnet_helm_table_write / the package's table generator). */
+ * table as written by the package's table generator (sph-exa-nnet_amd/helm/helm_table_gen). */
 int nnet_ctx_create(nnet_ctx** ctx, int device, const char* helm_table_path);
 void nnet_ctx_destroy(nnet_ctx* ctx);
 const char* nnet_last_error(const nnet_ctx* ctx);

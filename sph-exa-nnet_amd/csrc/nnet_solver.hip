@@ -81,10 +81,10 @@ struct NetDev
     //   row_ptr16[d+1], nz_col8[nnz]: non-zeros of row a are [row_ptr[a], row_ptr[a+1]), columns
     //                 ascending
     const uint32_t* img;
-    int             img_words, oRx, oEnt16, oSpPtr16, oSpEnt, oNzPtr16, oNzEnt, oRowPtr16, oNzCol8;
+    int             img_words, oRx, oEnt16, oSpPtr16, oSpEnt, oNzPtr16, oNzEnt, oRowPtr16, oRowEnt;
     // per lane, per register slot: index of the Jacobian non-zero the slot holds (-1: zero),
     // laid out [slot][lane] for coalesced loads
-    const int16_t* slotmap;
+    const uint4*   lanemap; // [lane]: register-slot mask (x, y), first non-zero id (z)
 };
 
 constexpr int MAXSP = 88; // species + 1 of the largest network
@@ -149,12 +149,12 @@ struct Cfg<14>
 template<>
 struct Cfg<86>
 {
-    static constexpr int D = 86, N = 87, NR = 309, NE = 157, NT = 256, TT = 6, WR = 2, NNZ = 800, IMG = 4224, WPE = 2;
+    static constexpr int D = 86, N = 87, NR = 309, NE = 157, NT = 256, TT = 6, WR = 2, NNZ = 800, IMG = 4736, WPE = 2;
 };
 template<>
 struct Cfg<87>
 {
-    static constexpr int D = 87, N = 88, NR = 311, NE = 157, NT = 256, TT = 6, WR = 2, NNZ = 800, IMG = 4224, WPE = 2;
+    static constexpr int D = 87, N = 88, NR = 311, NE = 157, NT = 256, TT = 6, WR = 2, NNZ = 800, IMG = 4736, WPE = 2;
 };
 
 template<int NET>
@@ -182,10 +182,20 @@ struct Lds
 // wc + WR*tj).  Elimination proceeds in blocks of four pivots (eigen.hpp:182-206 order per
 // element): the owners publish the block's four columns and four rows to LDS, every wave factors
 // the 4x4 diagonal block redundantly, derives its multipliers (A operand, rows) and normalised
-// pivot-row entries (B operand, columns) with the reference's 1e-40 skip test, and applies the
+// pivot-row entries (B operand, columns) with the reference's 1e-40 skip test (a skipped
+// multiplier is zero, so its update adds zero), and applies the
 // rank-4 trailing update to its tiles with one MFMA per tile.  One barrier per block.
 // ---------------------------------------------------------------------------------------------
 typedef double dx4 __attribute__((ext_vector_type(4)));
+
+// value of x in lane `lane` of the wavefront (lane is wave-uniform)
+__device__ __forceinline__ double bcast_lane(double x, int lane)
+{
+    const unsigned long long b  = __double_as_longlong(x);
+    const unsigned           lo = __builtin_amdgcn_readlane((unsigned)b, lane);
+    const unsigned           hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 template<int NET>
 struct TileLU
@@ -259,7 +269,7 @@ struct TileLU
             for (int r = q + 1; r < nq; ++r)
             {
                 const double w = Dg[r][q];
-                Wt[r][q]       = w;
+                Wt[r][q]       = fabs(w) > k::epsilonSystem ? w : 0.;
                 if (fabs(w) > k::epsilonSystem)
 #pragma unroll
                     for (int c = q + 1; c < 4; ++c)
@@ -289,14 +299,11 @@ struct TileLU
 #pragma unroll
                 for (int q = 0; q < nq; ++q)
                 {
-                    const double w = pr[q];
-                    if (fabs(w) > k::epsilonSystem)
-                    {
-                        Lq[q] = w;
+                    const double w = fabs(pr[q]) > k::epsilonSystem ? pr[q] : 0.;
+                    Lq[q]          = w;
 #pragma unroll
-                        for (int c = q + 1; c < nq; ++c)
-                            pr[c] = fma(-w, Un[q][c], pr[c]);
-                    }
+                    for (int c = q + 1; c < nq; ++c)
+                        pr[c] = fma(-w, Un[q][c], pr[c]);
                 }
                 Aop[ti] = -pick4(Lq, lr);
             }
@@ -320,7 +327,7 @@ struct TileLU
                     u[q] *= rinv[q];
 #pragma unroll
                     for (int r = q + 1; r < nq; ++r)
-                        if (fabs(Wt[r][q]) > k::epsilonSystem) u[r] = fma(-Wt[r][q], u[q], u[r]);
+                        u[r] = fma(-Wt[r][q], u[q], u[r]);
                 }
                 const double mine = lr < nq ? pick4(u, lr) : 0.;
                 Bop[tj]           = mine;
@@ -414,7 +421,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
     const uint16_t* nzp16  = reinterpret_cast<const uint16_t*>(simg) + nd.oNzPtr16;
     const uint32_t* nzent  = simg + nd.oNzEnt;
     const uint16_t* rowp16 = reinterpret_cast<const uint16_t*>(simg) + nd.oRowPtr16;
-    const uint8_t*  nzcol  = reinterpret_cast<const uint8_t*>(simg) + nd.oNzCol8;
+    const uint32_t* rowent = simg + nd.oRowEnt;
 
     for (;;)
     {
@@ -706,56 +713,104 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             //      (derivativesFromReactions), each summed in the reference's order ----
             for (int e = tid; e < nd.nnz; e += NT)
             {
+                // J = sum over the entry's contributions in list order of count * mass-action
+                // derivative, skipping reactions whose derivative factor is <= 1e-40; eight
+                // contributions are loaded ahead of their (sequential) additions
                 double    J  = 0.;
-                const int k0 = nzp16[e], k1 = nzp16[e + 1];
-                for (int kk = k0; kk < k1; ++kk)
+                int       kk = nzp16[e];
+                const int k1 = nzp16[e + 1];
+                for (; kk + 8 <= k1; kk += 8)
                 {
-                    const uint32_t w = nzent[kk];
-                    const int      r = w & 0xfff;
-                    if (sJok[r] == 0.) continue;
-                    const int    c = (int8_t)(w >> 16);
-                    const double t = sTr[r * MAXQ + ((w >> 12) & 0xf)];
-                    if (c < 0) J -= t * (double)(-c);
-                    else
-                        J += t * (double)c;
+                    double t[8];
+                    bool   ok[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                    {
+                        const uint32_t w = nzent[kk + u];
+                        const int      r = w & 0xfff;
+                        ok[u]            = sJok[r] != 0.;
+                        t[u]             = sTr[r * MAXQ + ((w >> 12) & 0xf)] * (double)(int8_t)(w >> 16);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        J = ok[u] ? J + t[u] : J;
+                }
+                for (; kk < k1; ++kk)
+                {
+                    const uint32_t w  = nzent[kk];
+                    const int      r  = w & 0xfff;
+                    const double   t  = sTr[r * MAXQ + ((w >> 12) & 0xf)] * (double)(int8_t)(w >> 16);
+                    J                 = sJok[r] != 0. ? J + t : J;
                 }
                 sJv[e] = J;
             }
             {
+                // dY/dt and d(dY/dt)/dT per species: signed-count sums in list order
                 const int s = tid - (NT - D);
                 if (s >= 0)
                 {
-                    double    a = 0., b = 0.;
-                    const int e0 = spp16[s], e1 = spp16[s + 1];
-                    for (int e = e0; e < e1; ++e)
+                    double a = 0., b = 0.;
+                    int    e = spp16[s];
+                    const int e1 = spp16[s + 1];
+                    for (; e + 8 <= e1; e += 8)
+                    {
+                        double x[8], y[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                        {
+                            const uint32_t w = spent[e + u];
+                            const int      r = w & 0xffff;
+                            const double   c = (double)(int8_t)(w >> 16);
+                            x[u]             = sVal[r] * c;
+                            y[u]             = sDval[r] * c;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                        {
+                            a = a + x[u];
+                            b = b + y[u];
+                        }
+                    }
+                    for (; e < e1; ++e)
                     {
                         const uint32_t w = spent[e];
-                        const int      r = w & 0xffff, c = (int8_t)(w >> 16);
-                        if (c < 0)
-                        {
-                            a -= sVal[r] * (double)(-c);
-                            b -= sDval[r] * (double)(-c);
-                        }
-                        else
-                        {
-                            a += sVal[r] * (double)c;
-                            b += sDval[r] * (double)c;
-                        }
+                        const int      r = w & 0xffff;
+                        const double   c = (double)(int8_t)(w >> 16);
+                        a                = a + sVal[r] * c;
+                        b                = b + sDval[r] * c;
                     }
                     sDYd[s] = a;
                     sDYT[s] = b;
                 }
             }
+            // this lane's register-slot mask and first non-zero id (for the matrix fill)
+            const uint4 lmap = nd.lanemap[tid];
             __syncthreads();
             // RHS correction sum_j J[a][j] (Yg[j] - Y[j]) over the row's non-zeros, j ascending
             if (tid < D)
             {
                 double    corr = 0;
-                const int e0 = rowp16[tid], e1 = rowp16[tid + 1];
-                for (int e = e0; e < e1; ++e)
+                int       e    = rowp16[tid];
+                const int e1   = rowp16[tid + 1];
+                for (; e + 8 <= e1; e += 8)
                 {
-                    const int b = nzcol[e];
-                    corr += sJv[e] * (sYb[b] - sY[b]);
+                    double x[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                    {
+                        const uint32_t w = rowent[e + u];
+                        const int      b = w >> 16;
+                        x[u]             = sJv[w & 0xffff] * (sYb[b] - sY[b]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        corr = corr + x[u];
+                }
+                for (; e < e1; ++e)
+                {
+                    const uint32_t w = rowent[e];
+                    const int      b = w >> 16;
+                    corr             = corr + sJv[w & 0xffff] * (sYb[b] - sY[b]);
                 }
                 sCorr[tid] = corr;
             }
@@ -779,35 +834,37 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             const double drho    = drho_dt * used_dt;
             const double value_1 = dpdTv * drho / (rho_t * rho_t);
             dx4 Mt[TI][TI];
+            {
+                const uint64_t mask = (uint64_t)lmap.x | ((uint64_t)lmap.y << 32);
+                int            nzi  = (int)lmap.z; // id of this lane's next non-zero
 #pragma unroll
-            for (int ti = 0; ti < TI; ++ti)
+                for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-                for (int tj = 0; tj < TI; ++tj)
+                    for (int tj = 0; tj < TI; ++tj)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                    {
-                        const int ri = 16 * (wr + WR * ti) + lr + 4 * v, ci = 16 * (wc + WR * tj) + lc;
-                        const int nzi = nd.slotmap[((ti * TI + tj) * 4 + v) * NT + tid];
-                        double    x   = 0.;
-                        if (ri < N && ci <= N)
+                        for (int v = 0; v < 4; ++v)
                         {
-                            if (ri == 0)
+                            const int  slot = (ti * TI + tj) * 4 + v;
+                            const int  ri = 16 * (wr + WR * ti) + lr + 4 * v, ci = 16 * (wc + WR * tj) + lc;
+                            const bool nz = (mask >> slot) & 1;
+                            const double J = nz ? sJv[nzi] : 0.;
+                            nzi += nz ? 1 : 0;
+                            double x = 0.;
+                            // row 0 (temperature) can only sit in register 0 of tile row 0
+                            if (ti == 0 && v == 0 && (wr | lr) == 0)
                             {
                                 if (ci == 0) x = 1 - k::theta * value_1 / cvv;
                                 else if (ci < N)
                                     x = -sBE[ci - 1] / cvv;
-                                else
+                                else if (ci == N)
                                     x = T * value_1 / cvv;
                             }
-                            else
+                            else if (ri < N && ci <= N)
                             {
                                 const int s = ri - 1;
                                 if (ci == 0) x = sDYT[s] * mthdt;
                                 else if (ci < N)
-                                {
-                                    const double J = nzi >= 0 ? sJv[nzi] : 0.;
-                                    x              = (ci == ri) ? 1. - thdt * J : mthdt * J;
-                                }
+                                    x = (ci == ri) ? 1. - thdt * J : mthdt * J;
                                 else
                                 {
                                     double rhs = sDYd[s] * used_dt;
@@ -816,44 +873,55 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                                     x = rhs;
                                 }
                             }
+                            Mt[ti][tj][v] = x;
                         }
-                        Mt[ti][tj][v] = x;
-                    }
+            }
             __syncthreads(); // the packed rows below alias the assembly arrays
 
             // ---- LU (eigen.hpp:179-206), four pivots per block ----
             TileLU<NET>::factor(Mt, sPan, sU, wr, wc, lr, lc, tid);
 
             // ---- back substitution on the packed unit-upper rows (eigen.hpp:208-221), 16-row
-            //      blocks from the bottom: one wavefront solves the block, all lanes update the
-            //      rows above ----
+            //      blocks from the bottom: lanes 0..15 of wavefront 0 solve the block with lane
+            //      broadcasts, then every lane updates one row above the block ----
             {
-                constexpr int BS = 16;
-                double*       sAcc = sPan; // the panel buffers are free now
+                double* sAcc = sPan; // the panel buffers are free now
                 for (int r = tid; r < N; r += NT)
                     sAcc[r] = sU[TileLU<NET>::urow(r) + N];
                 __syncthreads();
-                for (int r1 = N; r1 > 0; r1 -= BS)
+                for (int r1 = N; r1 > 0; r1 -= 16)
                 {
-                    const int r0 = r1 - BS > 0 ? r1 - BS : 0;
+                    const int r0 = r1 - 16 > 0 ? r1 - 16 : 0;
                     if (tid < 64)
                     {
                         const int row = r0 + tid;
-                        double    acc = (row < r1) ? sAcc[row] : 0.;
-                        for (int j = r1 - 1; j >= r0; --j)
+                        const bool own = row < r1;
+                        double    acc = own ? sAcc[row] : 0.;
+                        double    uu[16];
+#pragma unroll
+                        for (int jj = 0; jj < 16; ++jj)
+                            uu[jj] = (own && r0 + jj > row && r0 + jj < r1) ? sU[TileLU<NET>::urow(row) + r0 + jj] : 0.;
+#pragma unroll
+                        for (int jj = 15; jj >= 0; --jj)
                         {
-                            const double xj = __shfl(acc, j - r0, 64);
-                            if (row < j) acc = fma(-sU[TileLU<NET>::urow(row) + j], xj, acc);
-                            if (tid == j - r0) sX[j] = xj;
+                            if (r0 + jj >= r1) continue;
+                            const double xj = bcast_lane(acc, jj);
+                            acc             = acc - uu[jj] * xj;
+                            if (tid == jj) sX[r0 + jj] = xj;
                         }
                     }
                     __syncthreads();
                     for (int row = tid; row < r0; row += NT)
                     {
-                        double        acc = sAcc[row];
-                        const double* Ur  = sU + TileLU<NET>::urow(row);
-                        for (int j = r1 - 1; j >= r0; --j)
-                            acc = fma(-Ur[j], sX[j], acc);
+                        const double* Ur = sU + TileLU<NET>::urow(row);
+                        double        x[16];
+#pragma unroll
+                        for (int jj = 0; jj < 16; ++jj)
+                            x[jj] = (r0 + jj < r1) ? Ur[r0 + jj] * sX[r0 + jj] : 0.;
+                        double acc = sAcc[row];
+#pragma unroll
+                        for (int jj = 15; jj >= 0; --jj)
+                            if (r0 + jj < r1) acc = acc - x[jj];
                         sAcc[row] = acc;
                     }
                     __syncthreads();
@@ -1203,15 +1271,29 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
             for (int o = nw->pb[r]; o < nw->rb[r + 1]; ++o)
                 ent[(size_t)nw->ids[o] * d + b].push_back({r, qb - nw->rb[r], nw->nums[o]});
         }
-    std::vector<int> nz_of((size_t)d * d, -1), nz_row, nz_col;
-    for (int a = 0; a < d; ++a)
-        for (int b = 0; b < d; ++b)
-            if (!ent[(size_t)a * d + b].empty())
-            {
-                nz_of[(size_t)a * d + b] = (int)nz_row.size();
-                nz_row.push_back(a);
-                nz_col.push_back(b);
-            }
+    // Non-zeros are numbered in the order of the register slots that hold them (lane-major, then
+    // slot), so each lane's entries are consecutive: a lane finds its k-th entry at its base + k.
+    const int NT = kind == 14 ? Cfg<14>::NT : Cfg<86>::NT, TT = kind == 14 ? Cfg<14>::TT : Cfg<86>::TT,
+              WR = kind == 14 ? Cfg<14>::WR : Cfg<86>::WR, TI = TT / WR;
+    std::vector<int>      nz_of((size_t)d * d, -1), nz_row, nz_col;
+    std::vector<uint32_t> lanemap((size_t)NT * 4, 0u); // per lane: slot mask (lo, hi), base, unused
+    for (int t = 0; t < NT; ++t)
+    {
+        const int l = t & 63, wr = (t >> 6) / WR, wc = (t >> 6) % WR;
+        lanemap[(size_t)t * 4 + 2] = (uint32_t)nz_row.size();
+        for (int ti = 0; ti < TI; ++ti)
+            for (int tj = 0; tj < TI; ++tj)
+                for (int v = 0; v < 4; ++v)
+                {
+                    const int ri = 16 * (wr + WR * ti) + (l >> 4) + 4 * v, ci = 16 * (wc + WR * tj) + (l & 15);
+                    if (ri < 1 || ri > d || ci < 1 || ci > d || ent[(size_t)(ri - 1) * d + (ci - 1)].empty()) continue;
+                    const int slot = (ti * TI + tj) * 4 + v;
+                    lanemap[(size_t)t * 4 + (slot >> 5)] |= 1u << (slot & 31);
+                    nz_of[(size_t)(ri - 1) * d + (ci - 1)] = (int)nz_row.size();
+                    nz_row.push_back(ri - 1);
+                    nz_col.push_back(ci - 1);
+                }
+    }
     const int nnz = (int)nz_row.size();
 
     std::vector<uint32_t> img;
@@ -1268,45 +1350,25 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
     for (int e = 0; e < nnz; ++e)
         for (auto& x : ent[(size_t)nz_row[e] * d + nz_col[e]])
             img.push_back((uint32_t)x.r | ((uint32_t)x.q << 12) | (((uint32_t)(uint8_t)(int8_t)x.c) << 16));
+    // per row, its non-zeros in ascending column order: (id, column)
     const int oRowPtr16 = (int)img.size() * 2;
     h.clear();
+    std::vector<uint32_t> rowent;
+    for (int a = 0; a < d; ++a)
     {
-        int e = 0;
-        for (int a = 0; a <= d; ++a)
-        {
-            while (e < nnz && nz_row[e] < a)
-                ++e;
-            h.push_back((uint16_t)e);
-        }
+        h.push_back((uint16_t)rowent.size());
+        for (int b = 0; b < d; ++b)
+            if (nz_of[(size_t)a * d + b] >= 0) rowent.push_back((uint32_t)nz_of[(size_t)a * d + b] | ((uint32_t)b << 16));
     }
+    h.push_back((uint16_t)rowent.size());
     push_half(h);
-    const int            oNzCol8 = (int)img.size() * 4;
-    std::vector<uint8_t> c8(nz_col.begin(), nz_col.end());
-    while (c8.size() % 4)
-        c8.push_back(0);
-    for (size_t i = 0; i < c8.size(); i += 4)
-        img.push_back((uint32_t)c8[i] | ((uint32_t)c8[i + 1] << 8) | ((uint32_t)c8[i + 2] << 16) |
-                      ((uint32_t)c8[i + 3] << 24));
+    const int oRowEnt = (int)img.size();
+    img.insert(img.end(), rowent.begin(), rowent.end());
 
     const int img_cap = kind == 14 ? Cfg<14>::IMG : Cfg<86>::IMG;
     const int nnz_cap = kind == 14 ? Cfg<14>::NNZ : Cfg<86>::NNZ;
-    if ((int)img.size() > img_cap || nnz > nnz_cap || nw->rb[nR] > 65535)
+    if ((int)img.size() > img_cap || nnz >= nnz_cap || nw->rb[nR] > 65535)
         return fail(ctx, NNET_E_NETWORK, "reaction list too large for the kernel's LDS image");
-
-    // per-lane register slots -> non-zero index ([slot][lane])
-    const int NT = kind == 14 ? Cfg<14>::NT : Cfg<86>::NT, TT = kind == 14 ? Cfg<14>::TT : Cfg<86>::TT,
-              WR = kind == 14 ? Cfg<14>::WR : Cfg<86>::WR, TI = TT / WR;
-    std::vector<int16_t> slotmap((size_t)TI * TI * 4 * NT, -1);
-    for (int t = 0; t < NT; ++t)
-        for (int ti = 0; ti < TI; ++ti)
-            for (int tj = 0; tj < TI; ++tj)
-                for (int v = 0; v < 4; ++v)
-                {
-                    const int l = t & 63, wr = (t >> 6) / WR, wc = (t >> 6) % WR;
-                    const int ri = 16 * (wr + WR * ti) + (l >> 4) + 4 * v, ci = 16 * (wc + WR * tj) + (l & 15);
-                    if (ri >= 1 && ri <= d && ci >= 1 && ci <= d)
-                        slotmap[(size_t)((ti * TI + tj) * 4 + v) * NT + t] = (int16_t)nz_of[(size_t)(ri - 1) * d + (ci - 1)];
-                }
 
     // ---- upload ----
     Blob bl;
@@ -1315,7 +1377,7 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
            oScl = bl.add(scl), oPe = bl.add(pe), oPl = bl.add(pl), oElT = bl.add(NNET_ELEC_LOGT, NNET_ELEC_NT),
            oElR = bl.add(NNET_ELEC_LOGRHO, NNET_ELEC_NR),
            oElTab = bl.add(NNET_ELEC_RATE, (size_t)NNET_ELEC_NT * NNET_ELEC_NR * NNET_ELEC_NC), oImg = bl.add(img),
-           oSlot = bl.add(slotmap);
+           oLane = bl.add(lanemap);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipMalloc(&nw->blob, bl.b.size()));
     HIP_TRY(ctx, hipMemcpy(nw->blob, bl.b.data(), bl.b.size(), hipMemcpyHostToDevice));
@@ -1331,7 +1393,7 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
     n.etab = (double*)(B + oElTab), n.BEe = NNET87_ELECTRON_BE_MEV[0] * NNET_MEV_TO_ERG;
     n.img = (uint32_t*)(B + oImg), n.img_words = (int)img.size(), n.oRx = oRx, n.oEnt16 = oEnt16;
     n.oSpPtr16 = oSpPtr16, n.oSpEnt = oSpEnt, n.oNzPtr16 = oNzPtr16, n.oNzEnt = oNzEnt, n.oRowPtr16 = oRowPtr16;
-    n.oNzCol8 = oNzCol8, n.slotmap = (int16_t*)(B + oSlot);
+    n.oRowEnt = oRowEnt, n.lanemap = (const uint4*)(B + oLane);
     return NNET_OK;
 }
 

@@ -1,0 +1,87 @@
+"""The C++ mirror of computeNuclearReactions (sph-exa-nnet_amd/include/nnet_amd) driven the way
+SPH-EXA calls the reference (Data.devData fields, ReactionList, rate and EOS functors): its
+results equal the C-ABI batch path bit for bit and meet the oracle parity bar."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import workloads
+from parity import assert_parity, compare, summary
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "cpp", "shim_driver")
+TABLE = os.path.join(ROOT, "sph-exa-nnet_amd", "data", "helm_table.bin")
+
+
+def _write_input(path, cfg):
+    d = cfg["Y"].shape[0]
+    with open(path, "wb") as f:
+        np.array([cfg["rho"].size], np.int64).tofile(f)
+        np.array([cfg["hydro_dt"], cfg["mu"]], np.float64).tofile(f)
+        for a in (cfg["rho"], cfg["temp"], cfg["dt"], cfg["Y"]):
+            np.ascontiguousarray(a, np.float64).tofile(f)
+        if cfg["eos"] == 1:
+            Z = np.zeros(d)
+            Z[:len(cfg["Z"])] = cfg["Z"]
+            Z.tofile(f)
+
+
+def _read_output(path, n, d):
+    raw = open(path, "rb").read()
+    o = 0
+    T = np.frombuffer(raw, np.float64, n, o); o += 8 * n
+    dt = np.frombuffer(raw, np.float64, n, o); o += 8 * n
+    Y = np.frombuffer(raw, np.float64, d * n, o).reshape(d, n); o += 8 * d * n
+    st = np.frombuffer(raw, np.uint8, n, o); o += n
+    sub = np.frombuffer(raw, np.uint32, n, o); o += 4 * n
+    new = np.frombuffer(raw, np.uint32, n, o)
+    return Y, T, dt, st, sub, new
+
+
+@pytest.mark.parametrize("cid,n", [(2, 512), (3, 192)])
+def test_shim_matches_cabi_and_oracle(tmp_path, oracle_lib, cid, n):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    if not os.path.exists(DRIVER):
+        pytest.skip("tests/cpp/shim_driver not built (run __graft_entry__.build())")
+    import nnet_mi355x as nm
+    cfg = workloads.config(cid, n=n, seed=777 + cid)
+    d = cfg["Y"].shape[0]
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_input(fin, cfg)
+    print(f"shim_driver net{cfg['net']} n={n}", flush=True)
+    subprocess.run([DRIVER, str(cfg["net"]), str(cfg["eos"]), fin, fout, TABLE], check=True, timeout=120)
+    s = _read_output(fout, n, d)
+
+    ctx = nm.Context(0, helm_table=TABLE)
+    st = nm.device_state(torch, cfg["net"], cfg["rho"], cfg["temp"], cfg["Y"], cfg["dt"])
+    b = ctx.make_batch(cfg["net"], st, eos=cfg["eos"], mu=cfg["mu"], Z=cfg["Z"])
+    ctx.compute([b], cfg["hydro_dt"])
+    torch.cuda.synchronize()
+    g = (st["Y"].cpu().numpy(), st["temp"].cpu().numpy(), st["dt"].cpu().numpy(), st["status"].cpu().numpy(),
+         st["substeps"].cpu().numpy().astype(np.uint32), st["newton"].cpu().numpy().astype(np.uint32))
+    ctx.close()
+    for a, c in zip(s, g):
+        assert np.array_equal(np.asarray(a), np.asarray(c), equal_nan=True)
+
+    oracle_lib.helm_load(TABLE)
+    Y = np.ascontiguousarray(cfg["Y"].copy())
+    T, dt = cfg["temp"].copy(), cfg["dt"].copy()
+    so, subo, newo = oracle_lib.batch(cfg["net"], cfg["rho"].copy(), T, Y, dt, cfg["hydro_dt"], eos_kind=1, Z=cfg["Z"])
+    r = compare(workloads.mass_numbers(cfg["net"]), s[0], s[1], s[3], s[4], s[5], Y, T, so, subo, newo,
+                max_run_orc=oracle_lib.last_max_run)
+    print(cfg["label"], "shim vs oracle:", summary(r), flush=True)
+    oracle_lib.L.oracle_set_ulp_perturbation(1)
+    Yp = np.ascontiguousarray(cfg["Y"].copy())
+    Tp, dtp = cfg["temp"].copy(), cfg["dt"].copy()
+    sp, subp, newp = oracle_lib.batch(cfg["net"], cfg["rho"].copy(), Tp, Yp, dtp, cfg["hydro_dt"], eos_kind=1,
+                                      Z=cfg["Z"])
+    oracle_lib.L.oracle_set_ulp_perturbation(0)
+    rp = compare(workloads.mass_numbers(cfg["net"]), Yp, Tp, sp, subp, newp, Y, T, so, subo, newo,
+                 max_run_orc=oracle_lib.last_max_run)
+    assert_parity(r, rp)
