@@ -84,7 +84,22 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-sample", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (3 times): the reference kernel whose "
+                         "FETCH_SIZE/WRITE_SIZE counters calibrate the HBM traffic counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes,
+                                               "copies": 3}}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(particles are independent: N GPUs would each run their own share)", file=sys.stderr)

@@ -881,52 +881,35 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             // ---- LU (eigen.hpp:179-206), four pivots per block ----
             TileLU<NET>::factor(Mt, sPan, sU, wr, wc, lr, lc, tid);
 
-            // ---- back substitution on the packed unit-upper rows (eigen.hpp:208-221), 16-row
-            //      blocks from the bottom: lanes 0..15 of wavefront 0 solve the block with lane
-            //      broadcasts, then every lane updates one row above the block ----
+            // ---- back substitution on the packed unit-upper rows (eigen.hpp:208-221) by the
+            //      first wavefront: lane t holds rows t and t+64; x_j is broadcast with v_readlane
+            //      and subtracted from every row above it (column order, j descending) ----
+            if (tid < 64)
             {
-                double* sAcc = sPan; // the panel buffers are free now
-                for (int r = tid; r < N; r += NT)
-                    sAcc[r] = sU[TileLU<NET>::urow(r) + N];
-                __syncthreads();
-                for (int r1 = N; r1 > 0; r1 -= 16)
+                constexpr int NA = (N + 63) / 64;
+                double        acc[NA];
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
                 {
-                    const int r0 = r1 - 16 > 0 ? r1 - 16 : 0;
-                    if (tid < 64)
-                    {
-                        const int row = r0 + tid;
-                        const bool own = row < r1;
-                        double    acc = own ? sAcc[row] : 0.;
-                        double    uu[16];
-#pragma unroll
-                        for (int jj = 0; jj < 16; ++jj)
-                            uu[jj] = (own && r0 + jj > row && r0 + jj < r1) ? sU[TileLU<NET>::urow(row) + r0 + jj] : 0.;
-#pragma unroll
-                        for (int jj = 15; jj >= 0; --jj)
-                        {
-                            if (r0 + jj >= r1) continue;
-                            const double xj = bcast_lane(acc, jj);
-                            acc             = acc - uu[jj] * xj;
-                            if (tid == jj) sX[r0 + jj] = xj;
-                        }
-                    }
-                    __syncthreads();
-                    for (int row = tid; row < r0; row += NT)
-                    {
-                        const double* Ur = sU + TileLU<NET>::urow(row);
-                        double        x[16];
-#pragma unroll
-                        for (int jj = 0; jj < 16; ++jj)
-                            x[jj] = (r0 + jj < r1) ? Ur[r0 + jj] * sX[r0 + jj] : 0.;
-                        double acc = sAcc[row];
-#pragma unroll
-                        for (int jj = 15; jj >= 0; --jj)
-                            if (r0 + jj < r1) acc = acc - x[jj];
-                        sAcc[row] = acc;
-                    }
-                    __syncthreads();
+                    const int row = tid + 64 * a;
+                    acc[a]        = row < N ? sU[TileLU<NET>::urow(row) + N] : 0.;
                 }
+#pragma unroll
+                for (int j = N - 1; j > 0; --j)
+                {
+                    const double xj = bcast_lane(acc[j >> 6], j & 63);
+#pragma unroll
+                    for (int a = 0; a <= (j - 1) >> 6; ++a)
+                    {
+                        const int row = tid + 64 * a;
+                        if (row < j) acc[a] = fma(-sU[TileLU<NET>::urow(row) + j], xj, acc[a]);
+                    }
+                }
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+                    if (tid + 64 * a < N) sX[tid + 64 * a] = acc[a];
             }
+            __syncthreads();
             ++n_newton;
             ++total_iter;
 
