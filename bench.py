@@ -4,7 +4,10 @@
 Workload (default): net86 + Helmholtz EOS, 2,097,152 particles, 70% cool (T log-U[1e8,1e9] K,
 rho log-U[1e4,1e7] g/cm^3) / 30% hot (T log-U[3e9,8e9], rho log-U[1e7,5e9]), X(12C)=0.3,
 X(16O)=0.6, X(20Ne)=0.1, hydro dt = 1e-4 s per step, state resident on the device.  A "step" is
-one computeNuclearReactions call over all particles (one hydro step).
+one computeNuclearReactions call over all particles (one hydro step).  By default the plain FP64
+path and the mixed-precision path (FP32 tile LU on MFMA + FP64 iterative refinement, FP64 re-solve
+of iterations that do not refine) each run from the same seeded state; `value` is the mixed path's
+rate and config.mixed_over_fp64 the ratio.
 
 Prints one JSON line: particle-steps/s over the timed steps, the dominant kernel's roofline
 (algorithmic FLOPs from the device Newton-iteration counters: per iteration of an n = d+1
@@ -28,6 +31,13 @@ import workloads  # noqa: E402
 TABLE = os.path.join(ROOT, "sph-exa-nnet_amd", "data", "helm_table.bin")
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix)
 
+# HBM bytes per particle-step of the solve kernel on the net86 C3 distribution, from rocprofv3
+# --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of `bench.py --steps 1 --warmup 0 --no-cpu` (one
+# dispatch each), scaled by the same counters on a plain 512 MiB device copy
+# (`bench.py --copy-calibration 512`: FETCH_SIZE reads 0.5x and WRITE_SIZE 1.0x of the copied
+# bytes).  FP64 kernel: 130,068,949 KiB fetched + 142,210,817 KiB written per 2,097,152 particles.
+TRAFFIC_BYTES_PER_PARTICLE_STEP = {(3, "fp64"): (130068949 + 142210817) * 1024 / 2097152}
+
 
 def flops_per_iter(net):
     d = {14: 14, 86: 86, 87: 87}[net]
@@ -47,6 +57,7 @@ def device_config(torch, cid, n):
         "status": torch.zeros(n, dtype=torch.uint8, device=dev),
         "substeps": torch.zeros(n, dtype=torch.int32, device=dev),
         "newton": torch.zeros(n, dtype=torch.int32, device=dev),
+        "fp64_solves": torch.zeros(n, dtype=torch.int32, device=dev),
     }
     Y = torch.empty(cfg["Y"].shape, dtype=torch.float64, device=dev)
     for s in range(cfg["Y"].shape[0]):
@@ -77,11 +88,13 @@ def cpu_baseline(cid, sample, steps_hint):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3, choices=[2, 3, 4])
     ap.add_argument("--n", type=int, default=None, help="particles (default: the configuration's size)")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
+                    help="both: the FP64 path, then the mixed path, each from the same initial state; "
+                         "value is the mixed path's rate")
     ap.add_argument("--cpu-sample", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -109,59 +122,74 @@ def main():
     import nnet_mi355x as nm
 
     n = args.n or {2: 1048576, 3: 2097152, 4: 2097152}[args.config]
-    cfg, st = device_config(torch, args.config, n)
     ctx = nm.Context(0, helm_table=TABLE)
-    prec = nm.PRECISION_FP64 if args.precision == "fp64" else nm.PRECISION_MIXED
-    batch = ctx.make_batch(cfg["net"], st, eos=cfg["eos"], mu=cfg["mu"], Z=cfg["Z"], precision=prec)
     stream = torch.cuda.current_stream()
-    hdt = cfg["hydro_dt"]
-    print(f"# {cfg['label']}: n={n} precision={args.precision} warmup={args.warmup} steps={args.steps}",
-          file=sys.stderr, flush=True)
 
-    for w in range(args.warmup):
-        ctx.compute([batch], hdt, stream=stream.cuda_stream, sync=False)
+    def run(precision):
+        """Warmup + timed hydro steps of one precision from the seeded initial state."""
+        cfg, st = device_config(torch, args.config, n)
+        prec = nm.PRECISION_FP64 if precision == "fp64" else nm.PRECISION_MIXED
+        batch = ctx.make_batch(cfg["net"], st, eos=cfg["eos"], mu=cfg["mu"], Z=cfg["Z"], precision=prec)
+        hdt = cfg["hydro_dt"]
+        print(f"# {cfg['label']}: n={n} precision={precision} warmup={args.warmup} steps={args.steps}",
+              file=sys.stderr, flush=True)
+        for w in range(args.warmup):
+            ctx.compute([batch], hdt, stream=stream.cuda_stream, sync=False)
+            torch.cuda.synchronize()
+            print(f"# {precision} warmup step {w} done", file=sys.stderr, flush=True)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        newton_total = fp64_total = 0
+        status_hist = np.zeros(5, np.int64)
         torch.cuda.synchronize()
-        print(f"# warmup step {w} done", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            ev[k][0].record(stream)
+            ctx.compute([batch], hdt, stream=stream.cuda_stream, sync=False)
+            ev[k][1].record(stream)
+            # counters of this step (tiny D2H reads after the kernel on the same stream)
+            newton_total += int(st["newton"].sum().item())
+            fp64_total += int(st["fp64_solves"].sum().item())
+            status_hist += np.bincount(st["status"].cpu().numpy(), minlength=5)
+            print(f"# {precision} step {k} done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kernel_s = float(np.mean([x.elapsed_time(y) for x, y in ev])) / 1e3
+        flops_per_launch = newton_total / args.steps * flops_per_iter(cfg["net"])
+        return cfg, {"particle_steps_per_s": n * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
+                     "kernel_ms": kernel_s * 1e3, "achieved_tflops": flops_per_launch / kernel_s / 1e12,
+                     "newton_iters_per_step": newton_total / args.steps,
+                     "fp64_resolves_per_step": fp64_total / args.steps,
+                     "status_per_step": (status_hist / args.steps).tolist()}
 
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    newton_total = 0
-    status_hist = np.zeros(5, np.int64)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ev[k][0].record(stream)
-        ctx.compute([batch], hdt, stream=stream.cuda_stream, sync=False)
-        ev[k][1].record(stream)
-        # counters of this step (tiny D2H reads after the kernel on the same stream)
-        newton_total += int(st["newton"].sum().item())
-        status_hist += np.bincount(st["status"].cpu().numpy(), minlength=5)
-        print(f"# step {k} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = [a.elapsed_time(b) for a, b in ev]
-    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
-
-    value = n * args.steps / wall
-    flops_per_launch = newton_total / args.steps * flops_per_iter(cfg["net"])
-    achieved = flops_per_launch / avg_kernel_s
+    precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    res = {}
+    for p in precisions:
+        cfg, res[p] = run(p)
+    head = res["mixed"] if "mixed" in res else res["fp64"]
+    achieved = head["achieved_tflops"] * 1e12
+    head_prec = "mixed" if "mixed" in res else "fp64"
+    tb = TRAFFIC_BYTES_PER_PARTICLE_STEP.get((args.config, head_prec))
+    traffic = tb * n if tb is not None else None
+    conf = {"workload": cfg["label"], "particles": n, "hydro_dt": cfg["hydro_dt"], "precision": args.precision}
+    conf.update({p: r for p, r in res.items()})
+    if "mixed" in res and "fp64" in res:
+        conf["mixed_over_fp64"] = res["mixed"]["particle_steps_per_s"] / res["fp64"]["particle_steps_per_s"]
     out = {
         "metric": "net86 particle-steps/s, mixed vs FP64 on 1 GPU at <=1e-6 rel mass-frac error",
-        "value": value,
+        "value": head["particle_steps_per_s"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": cfg["label"], "particles": n, "hydro_dt": hdt, "precision": args.precision,
-                   "newton_iters_per_step": newton_total / args.steps,
-                   "status_per_step": (status_hist / args.steps).tolist()},
+        "config": conf,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None},
+                     "frac": achieved / PEAK_FP64, "traffic": traffic},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_sample, args.steps)

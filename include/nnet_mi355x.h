@@ -45,7 +45,8 @@ extern "C" {
 
 /* arithmetic of the Newton linear solve */
 #define NNET_PRECISION_FP64 0  /* plain FP64 everywhere (parity path) */
-#define NNET_PRECISION_MIXED 1 /* FP32 LU + FP64 iterative refinement */
+#define NNET_PRECISION_MIXED 1 /* FP32 LU on MFMA + FP64 iterative refinement (<= 3 sweeps,
+                                  FP64 re-solve of an iteration that does not converge) */
 
 /* per-particle status written to nnet_batch.status */
 #define NNET_STATUS_OK 0      /* integrated to hydro_dt */
@@ -104,6 +105,9 @@ typedef struct nnet_batch {
     uint8_t* status;      /* device, n, optional (NULL) */
     uint32_t* substeps;   /* device, n, optional: accepted substeps this call */
     uint32_t* newton;     /* device, n, optional: Newton iterations this call */
+    uint32_t* fp64_solves; /* device, n, optional: mixed precision only, Newton iterations solved
+                              in FP64: the first whose FP32 factors did not refine to FP64
+                              accuracy, and all later ones of that particle in this call */
 } nnet_batch;
 
 /* computeNuclearReactions for nb batches (different networks may share one call).

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/nnet_mi355x.h"
@@ -40,6 +41,10 @@ constexpr double epsilonSystem = 1e-40, epsilonVector = 1e-16;
 constexpr double NR_maxDt = 1e-2, NR_dTOverTempTarget = 1e-2, NR_dTOverTempTol = 4, NR_itTol = 1e-7;
 constexpr int    NR_maxIt      = 11;
 constexpr double substep_dtTol = 1e-6;
+// mixed-precision solve: at most this many FP64-residual refinement sweeps of the FP32 factors,
+// converged when every correction is below mixedTol of its component's scale
+constexpr int    mixedSweeps = 3;
+constexpr double mixedTol    = 1e-11;
 } // namespace k
 
 constexpr int MAXQ = 3; // reactant entries per reaction
@@ -110,6 +115,7 @@ struct BatchDev
     uint8_t*      status;
     uint32_t*     substeps;
     uint32_t*     newton;
+    uint32_t*     fp64_solves;
     unsigned int* queue; // work-queue counter (zeroed by nnet_queue_reset before the launch)
     double*       Y[MAXSP]; // per-species abundance arrays (SoA)
 };
@@ -157,22 +163,30 @@ struct Cfg<87>
     static constexpr int D = 87, N = 88, NR = 311, NE = 157, NT = 256, TT = 6, WR = 2, NNZ = 800, IMG = 4736, WPE = 2;
 };
 
-template<int NET>
+template<int NET, bool MIX>
 struct Lds
 {
     using C = Cfg<NET>;
     static constexpr int D = C::D, N = C::N, NR = C::NR, NE = C::NE;
-    static constexpr int NU = N * (N + 1) / 2; // packed normalised rows
-    // phase arrays (rates and assembly), aliased with the packed rows of the LU
-    static constexpr int oBE = 0, oMu = oBE + D, oEff = oMu + D, oDeff = oEff + NE, oL = oDeff + NE, oDl = oL + NE,
+    static constexpr int NU = N * (N + 1) / 2; // packed normalised rows (columns k+1 .. N)
+    static constexpr int NL = N * (N - 1) / 2; // packed multipliers (columns 0 .. k-1)
+    // head: rate and mass-action arrays, dead once the Jacobian is summed
+    static constexpr int oMu = 0, oEff = oMu + D, oDeff = oEff + NE, oL = oDeff + NE, oDl = oL + NE,
                          oRate = oDl + NE, oDrate = oRate + NR, oVal = oDrate + NR, oDval = oVal + NR,
-                         oTr = oDval + NR, oJok = oTr + NR * MAXQ, oDYdot = oJok + NR, oDYdT = oDYdot + D,
+                         oTr = oDval + NR, oJok = oTr + NR * MAXQ, headEnd = oJok + NR;
+    // FP32 factors of the mixed path (U rows, multipliers, pivot reciprocals), in doubles
+    static constexpr int f32Words = MIX ? (NU + NL + N + 1) / 2 : 0;
+    // tail: what the matrix fill and the FP64 refinement residual read
+    static constexpr int oBE = (headEnd > f32Words ? headEnd : f32Words), oDYdot = oBE + D, oDYdT = oDYdot + D,
                          oCorr = oDYdT + D, oEv = oCorr + D, oEos = oEv + 12, oJv = oEos + 4,
                          phaseEnd = oJv + C::NNZ;
+    // the FP64 packed rows alias everything above once the matrix sits in registers
     static constexpr int unionSize = phaseEnd > NU ? phaseEnd : NU;
-    static constexpr int oY = unionSize, oYb = oY + D, oX = oYb + D;
-    // LU panel buffers (two generations of: panel columns [row][4], pivot rows [column][4])
-    static constexpr int NPAD = 16 * C::TT, oPan = ((oX + N + 1) + 3) & ~3, total = oPan + 16 * NPAD;
+    static constexpr int oY = unionSize, oYb = oY + D, oX = oYb + D, oRow0 = oX + N + 1;
+    // LU panel buffers (two generations of: panel columns [row][4], pivot rows [column][4]); after
+    // the LU they hold the refinement residual
+    static constexpr int NPAD = 16 * C::TT, oPan = ((oRow0 + (MIX ? N + 1 : 0)) + 3) & ~3,
+                         total = oPan + 16 * NPAD;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -187,6 +201,7 @@ struct Lds
 // rank-4 trailing update to its tiles with one MFMA per tile.  One barrier per block.
 // ---------------------------------------------------------------------------------------------
 typedef double dx4 __attribute__((ext_vector_type(4)));
+typedef float  fx4 __attribute__((ext_vector_type(4)));
 
 // value of x in lane `lane` of the wavefront (lane is wave-uniform)
 __device__ __forceinline__ double bcast_lane(double x, int lane)
@@ -196,30 +211,62 @@ __device__ __forceinline__ double bcast_lane(double x, int lane)
     const unsigned           hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), lane);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
+__device__ __forceinline__ float bcast_lane(float x, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
 
-template<int NET>
+template<typename R>
+struct TileT;
+template<>
+struct TileT<double>
+{
+    using V = dx4;
+    // register v of lane l holds tile row (l>>4) + 4v: the A operand's row index l&15 is that row
+    static __device__ __forceinline__ int arow(int m) { return m; }
+    static __device__ __forceinline__ V   mfma(double a, double b, V c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template<>
+struct TileT<float>
+{
+    using V = fx4;
+    // the FP32 MFMA returns row 4(l>>4) + v in register v of lane l; the tiles keep the FP64
+    // element placement (row (l>>4) + 4v), so A-operand row m carries matrix row (m>>2) + 4(m&3)
+    static __device__ __forceinline__ int arow(int m) { return (m >> 2) + 4 * (m & 3); }
+    static __device__ __forceinline__ V   mfma(float a, float b, V c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template<int NET, typename R, bool STORE_L>
 struct TileLU
 {
     using C = Cfg<NET>;
+    using V = typename TileT<R>::V;
     static constexpr int N = C::N, TT = C::TT, WR = C::WR, TI = TT / WR, NPAD = 16 * TT;
     static constexpr int NS = (N + 3) / 4; // blocks of four pivots
     static_assert(TT % WR == 0 && WR * WR * 64 == C::NT && 16 * TT >= N + 1, "tile grid");
 
-    // first packed entry of normalised row k (columns k+1 .. N)
+    // first packed entry of normalised row k (columns k+1 .. N) / of multiplier row k (0 .. k-1)
     static __device__ __forceinline__ int urow(int k) { return k * N - k * (k - 1) / 2 - k - 1; }
+    static __device__ __forceinline__ int lrow(int k) { return k * (k - 1) / 2; }
 
-    static __device__ __forceinline__ double pick4(const double (&x)[4], int q)
+    static __device__ __forceinline__ R pick4(const R (&x)[4], int q)
     {
         return q == 0 ? x[0] : q == 1 ? x[1] : q == 2 ? x[2] : x[3];
     }
 
     // the owners of block STEP's columns and rows write them to the panel buffers
     template<int STEP>
-    static __device__ __forceinline__ void publish(const dx4 (&Mt)[TI][TI], double* pan, int wr, int wc, int lr, int lc)
+    static __device__ __forceinline__ void publish(const V (&Mt)[TI][TI], R* pan, int wr, int wc, int lr, int lc)
     {
         constexpr int p0 = 4 * STEP, Tp = p0 >> 4, c0 = p0 & 15, v0 = c0 >> 2;
-        double*       P  = pan + (STEP & 1) * 8 * NPAD; // [row][q]: columns p0 .. p0+3
-        double*       RT = P + 4 * NPAD;                // [column][q]: rows p0 .. p0+3
+        R*            P  = pan + (STEP & 1) * 8 * NPAD; // [row][q]: columns p0 .. p0+3
+        R*            RT = P + 4 * NPAD;                // [column][q]: rows p0 .. p0+3
         if (wc == Tp % WR && lc >= c0 && lc < c0 + 4)
         {
             constexpr int tj = Tp / WR;
@@ -240,37 +287,38 @@ struct TileLU
     }
 
     template<int STEP>
-    static __device__ __forceinline__ void step(dx4 (&Mt)[TI][TI], double* pan, double* sU, int wr, int wc, int lr,
-                                                int lc, int tid)
+    static __device__ __forceinline__ void step(V (&Mt)[TI][TI], R* pan, R* sU, R* sL, R* sRinv, int wr, int wc,
+                                                int lr, int lc, int tid)
     {
         constexpr int p0 = 4 * STEP, Tp = p0 >> 4, nq = (N - p0) < 4 ? (N - p0) : 4;
-        const double* P  = pan + (STEP & 1) * 8 * NPAD;
-        const double* RT = P + 4 * NPAD;
+        const R       eps = (R)k::epsilonSystem;
+        const R*      P   = pan + (STEP & 1) * 8 * NPAD;
+        const R*      RT  = P + 4 * NPAD;
 
         // 4x4 diagonal block (columns p0 .. p0+3 may include the right-hand side in the last block)
-        double Dg[4][4], Un[4][4], Wt[4][4], rinv[4];
+        R Dg[4][4], Un[4][4], Wt[4][4], rinv[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int c = 0; c < 4; ++c)
             {
                 Dg[r][c] = P[(p0 + r) * 4 + c];
-                Un[r][c] = 0.;
-                Wt[r][c] = 0.;
+                Un[r][c] = 0;
+                Wt[r][c] = 0;
             }
 #pragma unroll
         for (int q = 0; q < nq; ++q)
         {
-            rinv[q] = 1. / Dg[q][q];
+            rinv[q] = R(1) / Dg[q][q];
 #pragma unroll
             for (int c = q + 1; c < 4; ++c)
                 Un[q][c] = Dg[q][c] * rinv[q];
 #pragma unroll
             for (int r = q + 1; r < nq; ++r)
             {
-                const double w = Dg[r][q];
-                Wt[r][q]       = fabs(w) > k::epsilonSystem ? w : 0.;
-                if (fabs(w) > k::epsilonSystem)
+                const R w = Dg[r][q];
+                Wt[r][q]  = fabs(w) > eps ? w : R(0);
+                if (fabs(w) > eps)
 #pragma unroll
                     for (int c = q + 1; c < 4; ++c)
                         Dg[r][c] = fma(-w, Un[q][c], Dg[r][c]);
@@ -279,45 +327,57 @@ struct TileLU
         if (tid == 0)
 #pragma unroll
             for (int q = 0; q < nq; ++q)
+            {
 #pragma unroll
                 for (int c = q + 1; c < 4; ++c)
                     if (p0 + c <= N) sU[urow(p0 + q) + p0 + c] = Un[q][c];
+                if constexpr (STORE_L)
+                {
+                    sRinv[p0 + q] = rinv[q];
+#pragma unroll
+                    for (int c = 0; c < q; ++c)
+                        sL[lrow(p0 + q) + p0 + c] = Wt[q][c];
+                }
+            }
 
-        // multipliers of this lane's rows (A operand: row l&15 of the tile, pivot l>>4), negated
-        double Aop[TI];
+        // multipliers of this lane's rows (A operand: row arow(l&15) of the tile, pivot l>>4), negated
+        R Aop[TI];
 #pragma unroll
         for (int ti = 0; ti < TI; ++ti)
         {
-            const int I = wr + WR * ti, i = 16 * I + lc;
-            Aop[ti]     = 0.;
+            const int I = wr + WR * ti, i = 16 * I + TileT<R>::arow(lc);
+            Aop[ti]     = 0;
             if (I >= Tp && i >= p0 + 4 && i < N)
             {
-                double pr[4], Lq[4] = {0., 0., 0., 0.};
+                R pr[4], Lq[4] = {0, 0, 0, 0};
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     pr[c] = P[i * 4 + c];
 #pragma unroll
                 for (int q = 0; q < nq; ++q)
                 {
-                    const double w = fabs(pr[q]) > k::epsilonSystem ? pr[q] : 0.;
-                    Lq[q]          = w;
+                    const R w = fabs(pr[q]) > eps ? pr[q] : R(0);
+                    Lq[q]     = w;
 #pragma unroll
                     for (int c = q + 1; c < nq; ++c)
                         pr[c] = fma(-w, Un[q][c], pr[c]);
                 }
-                Aop[ti] = -pick4(Lq, lr);
+                const R mine = pick4(Lq, lr);
+                Aop[ti]      = -mine;
+                if constexpr (STORE_L)
+                    if (wc == 0 && lr < nq) sL[lrow(i) + p0 + lr] = mine;
             }
         }
         // normalised pivot-row entries of this lane's columns (B operand: pivot l>>4, column l&15)
-        double Bop[TI];
+        R Bop[TI];
 #pragma unroll
         for (int tj = 0; tj < TI; ++tj)
         {
             const int J = wc + WR * tj, j = 16 * J + lc;
-            Bop[tj]     = 0.;
+            Bop[tj]     = 0;
             if (J >= Tp && j >= p0 + 4 && j <= N)
             {
-                double u[4];
+                R u[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     u[q] = RT[j * 4 + q];
@@ -329,8 +389,8 @@ struct TileLU
                     for (int r = q + 1; r < nq; ++r)
                         u[r] = fma(-Wt[r][q], u[q], u[r]);
                 }
-                const double mine = lr < nq ? pick4(u, lr) : 0.;
-                Bop[tj]           = mine;
+                const R mine = lr < nq ? pick4(u, lr) : R(0);
+                Bop[tj]      = mine;
                 if (wr == 0 && lr < nq) sU[urow(p0 + lr) + j] = mine;
             }
         }
@@ -342,20 +402,56 @@ struct TileLU
             {
                 const int I = wr + WR * ti, J = wc + WR * tj;
                 if (I >= Tp && J >= Tp && 16 * I + 15 >= p0 + 4 && 16 * J + 15 >= p0 + 4)
-                    Mt[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(Aop[ti], Bop[tj], Mt[ti][tj], 0, 0, 0);
+                    Mt[ti][tj] = TileT<R>::mfma(Aop[ti], Bop[tj], Mt[ti][tj]);
             }
         if constexpr (STEP + 1 < NS) publish<STEP + 1>(Mt, pan, wr, wc, lr, lc);
         __syncthreads();
-        if constexpr (STEP + 1 < NS) step<STEP + 1>(Mt, pan, sU, wr, wc, lr, lc, tid);
+        if constexpr (STEP + 1 < NS) step<STEP + 1>(Mt, pan, sU, sL, sRinv, wr, wc, lr, lc, tid);
     }
 
-    // forward elimination of the augmented matrix; leaves the normalised rows in sU
-    static __device__ __forceinline__ void factor(dx4 (&Mt)[TI][TI], double* pan, double* sU, int wr, int wc, int lr,
-                                                  int lc, int tid)
+    // forward elimination of the augmented matrix; leaves the normalised rows in sU (and, with
+    // STORE_L, the multipliers in sL and the pivot reciprocals in sRinv)
+    static __device__ __forceinline__ void factor(V (&Mt)[TI][TI], R* pan, R* sU, R* sL, R* sRinv, int wr, int wc,
+                                                  int lr, int lc, int tid)
     {
         publish<0>(Mt, pan, wr, wc, lr, lc);
         __syncthreads();
-        step<0>(Mt, pan, sU, wr, wc, lr, lc, tid);
+        step<0>(Mt, pan, sU, sL, sRinv, wr, wc, lr, lc, tid);
+    }
+
+    // x = U^-1 c by the first wavefront, c = column N of the packed rows (or `rhs` if given):
+    // lane t holds rows t and t+64; x_j is broadcast with v_readlane and subtracted from every
+    // row above it (column order, j descending); returns this lane's rows in acc
+    static __device__ __forceinline__ void back(const R* sU, R (&acc)[(N + 63) / 64], int lane)
+    {
+#pragma unroll
+        for (int j = N - 1; j > 0; --j)
+        {
+            const R xj = bcast_lane(acc[j >> 6], j & 63);
+#pragma unroll
+            for (int a = 0; a <= (j - 1) >> 6; ++a)
+            {
+                const int row = lane + 64 * a;
+                if (row < j) acc[a] = fma(-sU[urow(row) + j], xj, acc[a]);
+            }
+        }
+    }
+    // y = forward elimination of r with the stored multipliers (eigen.hpp:182-199 applied to a new
+    // right-hand side): y_j = (r_j - sum_{k<j} L_jk y_k) / d_j, in column order
+    static __device__ __forceinline__ void forward(const R* sL, const R* sRinv, R (&acc)[(N + 63) / 64], int lane)
+    {
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+        {
+            const R yj = bcast_lane(acc[j >> 6], j & 63) * sRinv[j];
+            if ((j & 63) == lane) acc[j >> 6] = yj;
+#pragma unroll
+            for (int a = (j + 1) >> 6; a < (N + 63) / 64; ++a)
+            {
+                const int row = lane + 64 * a;
+                if (row > j && row < N) acc[a] = fma(-sL[lrow(row) + j], yj, acc[a]);
+            }
+        }
     }
 };
 
@@ -372,18 +468,18 @@ __device__ __forceinline__ int special_task(int tid)
 // The FP64 kernel: persistent workgroups take particles from a work queue; one workgroup
 // integrates one particle over one hydro step at a time.
 // ---------------------------------------------------------------------------------------------
-template<int NET, int EOSK>
+template<int NET, int EOSK, bool MIX>
 __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
-    nnet_fp64_kernel(NetDev nd, EosDev ed, BatchDev bd, StepParams sp)
+    nnet_solve_kernel(NetDev nd, EosDev ed, BatchDev bd, StepParams sp)
 {
     using C  = Cfg<NET>;
-    using L  = Lds<NET>;
-    constexpr int D = C::D, N = C::N, NT = C::NT, WR = C::WR, TI = C::TT / C::WR;
+    using L  = Lds<NET, MIX>;
+    constexpr int D = C::D, N = C::N, NT = C::NT, WR = C::WR, TI = C::TT / C::WR, NA = (N + 63) / 64;
     constexpr int Dr = (NET == 14) ? 14 : 86; // species of the rate functor's BE/screening loops
 
     __shared__ __align__(16) double sm[L::total];
     __shared__ uint32_t simg[C::IMG];
-    __shared__ unsigned s_p;
+    __shared__ unsigned s_p, s_flag;
     double* sBE   = sm + L::oBE;
     double* sMu   = sm + L::oMu;
     double* sEff  = sm + L::oEff;
@@ -407,6 +503,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
     double* sYb   = sm + L::oYb;
     double* sX    = sm + L::oX;
     double* sPan  = sm + L::oPan;
+    double* sRow0 = sm + L::oRow0;
 
     const int tid = threadIdx.x;
     const int lr = (tid & 63) >> 4, lc = tid & 15, wr = (tid >> 6) / WR, wc = (tid >> 6) % WR;
@@ -443,6 +540,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                 if (bd.status) bd.status[p] = NNET_STATUS_SKIPPED;
                 if (bd.substeps) bd.substeps[p] = 0;
                 if (bd.newton) bd.newton[p] = 0;
+                if (bd.fp64_solves) bd.fp64_solves[p] = 0;
             }
             continue;
         }
@@ -459,7 +557,8 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
         double       T_buf   = __builtin_nan("");
         double       att_dt = 0, att_Tb = 0;
         long         total_iter = 0;
-        uint32_t     n_sub = 0, n_newton = 0;
+        uint32_t     n_sub = 0, n_newton = 0, n_fallback = 0;
+        bool         fp64_only = false; // mixed path: FP32 refinement failed once for this particle
         uint8_t      status = NNET_STATUS_OK;
 
         for (int i = 1;; ++i)
@@ -833,7 +932,27 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             const double thdt    = k::theta * used_dt;
             const double drho    = drho_dt * used_dt;
             const double value_1 = dpdTv * drho / (rho_t * rho_t);
-            dx4 Mt[TI][TI];
+            // matrix entry (ri, ci) from the assembly arrays, as prepareSystemFromGuess builds it;
+            // J is the Jacobian entry (0 where the pattern has none)
+            auto entry = [&](int ri, int ci, double J) __attribute__((always_inline)) -> double
+            {
+                if (ri == 0)
+                {
+                    if (ci == 0) return 1 - k::theta * value_1 / cvv;
+                    if (ci < N) return -sBE[ci - 1] / cvv;
+                    return T * value_1 / cvv;
+                }
+                const int s = ri - 1;
+                if (ci == 0) return sDYT[s] * mthdt;
+                if (ci < N) return (ci == ri) ? 1. - thdt * J : mthdt * J;
+                double rhs = sDYd[s] * used_dt;
+                rhs += mthdt * sDYT[s] * (T_theta - T);
+                rhs += mthdt * sCorr[s];
+                return rhs;
+            };
+            // the lane's register slots (tile placement of TileLU) filled from the Jacobian
+            // non-zeros, which are numbered lane-major in slot order
+            auto fill = [&](auto& Mt) __attribute__((always_inline))
             {
                 const uint64_t mask = (uint64_t)lmap.x | ((uint64_t)lmap.y << 32);
                 int            nzi  = (int)lmap.z; // id of this lane's next non-zero
@@ -844,72 +963,163 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
 #pragma unroll
                         for (int v = 0; v < 4; ++v)
                         {
-                            const int  slot = (ti * TI + tj) * 4 + v;
-                            const int  ri = 16 * (wr + WR * ti) + lr + 4 * v, ci = 16 * (wc + WR * tj) + lc;
-                            const bool nz = (mask >> slot) & 1;
-                            const double J = nz ? sJv[nzi] : 0.;
+                            const int    slot = (ti * TI + tj) * 4 + v;
+                            const int    ri = 16 * (wr + WR * ti) + lr + 4 * v, ci = 16 * (wc + WR * tj) + lc;
+                            const bool   nz = (mask >> slot) & 1;
+                            const double J  = nz ? sJv[nzi] : 0.;
                             nzi += nz ? 1 : 0;
                             double x = 0.;
                             // row 0 (temperature) can only sit in register 0 of tile row 0
                             if (ti == 0 && v == 0 && (wr | lr) == 0)
                             {
-                                if (ci == 0) x = 1 - k::theta * value_1 / cvv;
-                                else if (ci < N)
-                                    x = -sBE[ci - 1] / cvv;
-                                else if (ci == N)
-                                    x = T * value_1 / cvv;
+                                if (ci <= N) x = entry(0, ci, 0.);
+                                if constexpr (MIX)
+                                    if (ci <= N) sRow0[ci] = x;
                             }
                             else if (ri < N && ci <= N)
-                            {
-                                const int s = ri - 1;
-                                if (ci == 0) x = sDYT[s] * mthdt;
-                                else if (ci < N)
-                                    x = (ci == ri) ? 1. - thdt * J : mthdt * J;
-                                else
-                                {
-                                    double rhs = sDYd[s] * used_dt;
-                                    rhs += mthdt * sDYT[s] * (T_theta - T);
-                                    rhs += mthdt * sCorr[s];
-                                    x = rhs;
-                                }
-                            }
+                                x = entry(ri, ci, J);
                             Mt[ti][tj][v] = x;
                         }
-            }
-            __syncthreads(); // the packed rows below alias the assembly arrays
-
-            // ---- LU (eigen.hpp:179-206), four pivots per block ----
-            TileLU<NET>::factor(Mt, sPan, sU, wr, wc, lr, lc, tid);
-
-            // ---- back substitution on the packed unit-upper rows (eigen.hpp:208-221) by the
-            //      first wavefront: lane t holds rows t and t+64; x_j is broadcast with v_readlane
-            //      and subtracted from every row above it (column order, j descending) ----
-            if (tid < 64)
+            };
+            auto solve_fp64 = [&]() __attribute__((always_inline))
             {
-                constexpr int NA = (N + 63) / 64;
-                double        acc[NA];
-#pragma unroll
-                for (int a = 0; a < NA; ++a)
+                dx4 Mt[TI][TI];
+                fill(Mt);
+                __syncthreads(); // the packed rows below alias the assembly arrays
+                // ---- LU (eigen.hpp:179-206), four pivots per block ----
+                TileLU<NET, double, false>::factor(Mt, sPan, sU, nullptr, nullptr, wr, wc, lr, lc, tid);
+                // ---- back substitution (eigen.hpp:208-221) ----
+                if (tid < 64)
                 {
-                    const int row = tid + 64 * a;
-                    acc[a]        = row < N ? sU[TileLU<NET>::urow(row) + N] : 0.;
-                }
+                    double acc[NA];
 #pragma unroll
-                for (int j = N - 1; j > 0; --j)
-                {
-                    const double xj = bcast_lane(acc[j >> 6], j & 63);
-#pragma unroll
-                    for (int a = 0; a <= (j - 1) >> 6; ++a)
+                    for (int a = 0; a < NA; ++a)
                     {
                         const int row = tid + 64 * a;
-                        if (row < j) acc[a] = fma(-sU[TileLU<NET>::urow(row) + j], xj, acc[a]);
+                        acc[a]        = row < N ? sU[TileLU<NET, double, false>::urow(row) + N] : 0.;
+                    }
+                    TileLU<NET, double, false>::back(sU, acc, tid);
+#pragma unroll
+                    for (int a = 0; a < NA; ++a)
+                        if (tid + 64 * a < N) sX[tid + 64 * a] = acc[a];
+                }
+                __syncthreads();
+            };
+
+            // the FP64 solve has a single (inlined) call site, so that every LDS access of the
+            // kernel stays a direct LDS instruction
+            bool use_fp64 = !MIX || fp64_only;
+            if constexpr (MIX)
+            {
+                if (!fp64_only)
+                {
+                    // ---- mixed precision: FP32 tile LU + FP64 iterative refinement; an iteration
+                    //      whose refinement does not converge is solved again in FP64, and so are the
+                    //      particle's remaining iterations in this call ----
+                    using LU32 = TileLU<NET, float, true>;
+                    float* sU32   = reinterpret_cast<float*>(sm);
+                    float* sL32   = sU32 + L::NU;
+                    float* sRi32  = sL32 + L::NL;
+                    float* sPan32 = reinterpret_cast<float*>(sPan);
+                    {
+                        fx4 Mt[TI][TI];
+                        fill(Mt);
+                        __syncthreads(); // FP32 factors alias the head arrays only
+                        LU32::factor(Mt, sPan32, sU32, sL32, sRi32, wr, wc, lr, lc, tid);
+                    }
+                    if (tid < 64)
+                    {
+                        float acc[NA];
+    #pragma unroll
+                        for (int a = 0; a < NA; ++a)
+                        {
+                            const int row = tid + 64 * a;
+                            acc[a]        = row < N ? sU32[LU32::urow(row) + N] : 0.f;
+                        }
+                        LU32::back(sU32, acc, tid);
+    #pragma unroll
+                        for (int a = 0; a < NA; ++a)
+                            if (tid + 64 * a < N) sX[tid + 64 * a] = (double)acc[a];
+                    }
+                    __syncthreads();
+                    double* sRes = sPan; // the panel buffers are free after the LU
+                    bool    conv = false;
+                    for (int sweep = 0; sweep < k::mixedSweeps && !conv; ++sweep)
+                    {
+                        // FP64 residual r = b - M x over the sparse pattern (row 0 is dense)
+                        for (int ri = tid; ri < N; ri += NT)
+                        {
+                            double r;
+                            if (ri == 0)
+                            {
+                                r = sRow0[N];
+                                for (int c = 0; c < N; ++c)
+                                    r = fma(-sRow0[c], sX[c], r);
+                            }
+                            else
+                            {
+                                const int s = ri - 1;
+                                r           = entry(ri, N, 0.);
+                                r           = fma(-entry(ri, 0, 0.), sX[0], r);
+                                bool diag   = false;
+                                for (int e = rowp16[s]; e < rowp16[s + 1]; ++e)
+                                {
+                                    const uint32_t w = rowent[e];
+                                    const int      c = (int)(w >> 16) + 1;
+                                    diag             = diag || c == ri;
+                                    r                = fma(-entry(ri, c, sJv[w & 0xffff]), sX[c], r);
+                                }
+                                if (!diag) r -= sX[ri];
+                            }
+                            sRes[ri] = r;
+                        }
+                        __syncthreads();
+                        // correction d = (LU)^-1 r in FP32; converged when every |d_i| is below
+                        // mixedTol of its magnitude scale (|x_0| + T for the temperature, |x_i| + Y_i
+                        // for the abundances)
+                        if (tid < 64)
+                        {
+                            float acc[NA];
+    #pragma unroll
+                            for (int a = 0; a < NA; ++a)
+                            {
+                                const int row = tid + 64 * a;
+                                acc[a]        = row < N ? (float)sRes[row] : 0.f;
+                            }
+                            LU32::forward(sL32, sRi32, acc, tid);
+                            LU32::back(sU32, acc, tid);
+                            bool ok = true;
+    #pragma unroll
+                            for (int a = 0; a < NA; ++a)
+                            {
+                                const int row = tid + 64 * a;
+                                if (row < N)
+                                {
+                                    const double d     = (double)acc[a];
+                                    const double xn    = sX[row] + d;
+                                    const double scale = fabs(xn) + (row == 0 ? T : fabs(sY[row - 1])) + 1e-30;
+                                    ok                 = ok && fabs(d) <= k::mixedTol * scale;
+                                    sX[row]            = xn;
+                                }
+                            }
+                            const bool all = __all(ok);
+                            if (tid == 0) s_flag = all ? 1u : 0u;
+                        }
+                        __syncthreads();
+                        conv = s_flag != 0;
+                    }
+                    if (!conv)
+                    {
+                        use_fp64  = true;
+                        fp64_only = true;
                     }
                 }
-#pragma unroll
-                for (int a = 0; a < NA; ++a)
-                    if (tid + 64 * a < N) sX[tid + 64 * a] = acc[a];
             }
-            __syncthreads();
+            if (use_fp64)
+            {
+                solve_fp64();
+                if constexpr (MIX) ++n_fallback;
+            }
             ++n_newton;
             ++total_iter;
 
@@ -1003,6 +1213,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             if (bd.status) bd.status[p] = status;
             if (bd.substeps) bd.substeps[p] = n_sub;
             if (bd.newton) bd.newton[p] = n_newton;
+            if (bd.fp64_solves) bd.fp64_solves[p] = n_fallback;
         }
     }
 }
@@ -1582,7 +1793,7 @@ extern "C"
             }
             BatchDev bd{};
             bd.n = b.n, bd.rho = b.rho, bd.rho_m1 = b.rho_m1, bd.temp = b.temp, bd.dt = b.dt;
-            bd.status = b.status, bd.substeps = b.substeps, bd.newton = b.newton;
+            bd.status = b.status, bd.substeps = b.substeps, bd.newton = b.newton, bd.fp64_solves = b.fp64_solves;
             for (int z = 0; z < d; ++z)
                 bd.Y[z] = b.Y[z];
             const NetDev& nd = b.network->dev;
@@ -1598,24 +1809,28 @@ extern "C"
                 hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(nt), 0, st, nd, ed, bd, sp);
                 return 0;
             };
-            if (kind == 14)
+            const bool mix      = b.precision == NNET_PRECISION_MIXED;
+            auto       dispatch = [&](auto netc)
             {
-                if (b.eos == NNET_EOS_IDEAL) launch(nnet_fp64_kernel<14, NNET_EOS_IDEAL>, Cfg<14>::NT);
+                constexpr int NV = decltype(netc)::value;
+                if (b.eos == NNET_EOS_IDEAL)
+                {
+                    if (mix) launch(nnet_solve_kernel<NV, NNET_EOS_IDEAL, true>, Cfg<NV>::NT);
+                    else
+                        launch(nnet_solve_kernel<NV, NNET_EOS_IDEAL, false>, Cfg<NV>::NT);
+                }
                 else
-                    launch(nnet_fp64_kernel<14, NNET_EOS_HELMHOLTZ>, Cfg<14>::NT);
-            }
+                {
+                    if (mix) launch(nnet_solve_kernel<NV, NNET_EOS_HELMHOLTZ, true>, Cfg<NV>::NT);
+                    else
+                        launch(nnet_solve_kernel<NV, NNET_EOS_HELMHOLTZ, false>, Cfg<NV>::NT);
+                }
+            };
+            if (kind == 14) dispatch(std::integral_constant<int, 14>{});
             else if (kind == 86)
-            {
-                if (b.eos == NNET_EOS_IDEAL) launch(nnet_fp64_kernel<86, NNET_EOS_IDEAL>, Cfg<86>::NT);
-                else
-                    launch(nnet_fp64_kernel<86, NNET_EOS_HELMHOLTZ>, Cfg<86>::NT);
-            }
+                dispatch(std::integral_constant<int, 86>{});
             else
-            {
-                if (b.eos == NNET_EOS_IDEAL) launch(nnet_fp64_kernel<87, NNET_EOS_IDEAL>, Cfg<87>::NT);
-                else
-                    launch(nnet_fp64_kernel<87, NNET_EOS_HELMHOLTZ>, Cfg<87>::NT);
-            }
+                dispatch(std::integral_constant<int, 87>{});
             HIP_TRY(c, hipGetLastError());
         }
         return NNET_OK;

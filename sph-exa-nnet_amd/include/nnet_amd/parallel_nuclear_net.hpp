@@ -283,6 +283,7 @@ struct Options
     uint8_t*  status    = nullptr; // device, lastIndex - firstIndex entries (optional)
     uint32_t* substeps  = nullptr; // device (optional)
     uint32_t* newton    = nullptr; // device (optional)
+    uint32_t* fp64_solves = nullptr; // device (optional, mixed precision)
 };
 
 template<bool use_gpu, class Data, typename Float, class List, int KIND, class Eos, class nseFunction = void*>
@@ -319,6 +320,7 @@ void computeNuclearReactions(Data& n, size_t firstIndex, size_t lastIndex, const
     b.status    = opt.status;
     b.substeps  = opt.substeps;
     b.newton    = opt.newton;
+    b.fp64_solves = opt.fp64_solves;
     if constexpr (std::is_same_v<Eos, eos::IdealGasFunctor>)
     {
         b.eos = NNET_EOS_IDEAL;

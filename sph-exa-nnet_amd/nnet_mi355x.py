@@ -39,6 +39,7 @@ class Batch(C.Structure):
         ("status", C.c_void_p),
         ("substeps", C.c_void_p),
         ("newton", C.c_void_p),
+        ("fp64_solves", C.c_void_p),
     ]
 
 
@@ -118,7 +119,8 @@ class Context:
         self.check(lib().nnet_ctx_set_max_iterations(self.h, m), "set_max_iterations")
 
     def make_batch(self, net, state, eos=EOS_IDEAL, mu=10.0, Z=None, precision=PRECISION_FP64):
-        """state: dict of torch device tensors rho, rho_m1, Y (d x n), temp, dt, status, substeps, newton."""
+        """state: dict of torch device tensors rho, rho_m1, Y (d x n), temp, dt, status, substeps, newton
+        and (optional) fp64_solves."""
         b = Batch()
         b.network = self.network(net).value
         b.eos = eos
@@ -138,6 +140,7 @@ class Context:
         ptr = lambda k: state[k].data_ptr() if state.get(k) is not None else None
         b.rho, b.rho_m1, b.temp, b.dt = ptr("rho"), ptr("rho_m1"), ptr("temp"), ptr("dt")
         b.status, b.substeps, b.newton = ptr("status"), ptr("substeps"), ptr("newton")
+        b.fp64_solves = ptr("fp64_solves")
         yp = (C.c_void_p * Y.shape[0])(*[Y[s].data_ptr() for s in range(Y.shape[0])])
         b.Y = C.cast(yp, C.POINTER(C.c_void_p))
         b._ykeep = yp
@@ -180,5 +183,6 @@ def device_state(torch, net, rho, temp, Y, dt, device="cuda:0"):
         "status": torch.zeros(n, dtype=torch.uint8, device=device),
         "substeps": torch.zeros(n, dtype=torch.int32, device=device),
         "newton": torch.zeros(n, dtype=torch.int32, device=device),
+        "fp64_solves": torch.zeros(n, dtype=torch.int32, device=device),
     }
     return st

@@ -47,6 +47,18 @@ def helmholtz_Z(net):
     return Z14.copy() if net == 14 else Z86[:86].copy() if net == 86 else Z86.copy()
 
 
+def binding_energies(net):
+    """Constant binding energies in erg/mol (BE in MeV x MeV-to-erg per mole), as the rate
+    functors use them before their Coulomb/thermal corrections."""
+    mev_to_erg = 9.648529392e17
+    if net == 14:
+        return np.array(_table("NNET14_BE_MEV")) * mev_to_erg
+    be = np.array(_table("NNET86_BE_MEV")) * mev_to_erg
+    if net == 87:
+        be = np.append(be, np.array(_table("NNET87_ELECTRON_BE_MEV")) * mev_to_erg)
+    return be
+
+
 def mass_numbers(net):
     return A14.copy() if net == 14 else A86[:86].copy() if net == 86 else A86.copy()
 

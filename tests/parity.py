@@ -100,3 +100,37 @@ def assert_parity(r, r_pert=None, tier1_min=TIER1_MIN_FRAC, counts_min=COUNTS_MI
 
 def summary(r):
     return {k: v for k, v in r.items() if k != "tier1_idx"}
+
+
+MIX_X_RTOL, MIX_T_RTOL = 1e-6, 1e-8
+
+
+def mixed_errors(A, BE, Y0, Y_gpu, T_gpu, st_gpu, Y_orc, T_orc, st_orc, max_run_orc=None):
+    """Per-output error of the mixed-precision path against the FP64 oracle, over the regular
+    particles (not chaotic, same status, not skipped): mass fractions X_i = Y_i A_i
+    (|dX| <= 1e-6 |X| + 1e-14), temperature (|dT| <= 1e-8 |T|) and the specific energy release
+    eps = sum_i BE_i (Y_i - Y0_i) [erg/g], BE the constant binding energies."""
+    chaotic = (st_orc == 4) | (st_gpu == 4)
+    if max_run_orc is not None:
+        chaotic |= max_run_orc > CHAOTIC_RUN
+    same = st_gpu == st_orc
+    reg = ~chaotic & same & (st_orc != 1)
+    A, BE = np.asarray(A), np.asarray(BE)
+    X_g, X_o = Y_gpu[:, reg] * A[:, None], Y_orc[:, reg] * A[:, None]
+    dx = np.abs(X_g - X_o)
+    dT = np.abs(T_gpu[reg] - T_orc[reg])
+    okx = np.all(dx <= MIX_X_RTOL * np.abs(X_o) + X_ATOL, axis=0)
+    okT = dT <= MIX_T_RTOL * np.abs(T_orc[reg])
+    eps_g = BE @ (Y_gpu[:, reg] - Y0[:, reg])
+    eps_o = BE @ (Y_orc[:, reg] - Y0[:, reg])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        relx = np.where(np.abs(X_o) > X_ATOL, dx / np.abs(X_o), 0.0)
+        rele = np.abs(eps_g - eps_o) / np.maximum(np.abs(eps_o), 1e-300)
+    n = int(reg.sum())
+    return dict(n=n, n_chaotic=int(chaotic.sum()), status_mismatch=int((~same & ~chaotic).sum()),
+                within=int((okx & okT).sum()),
+                max_relX=float(relx.max()) if n else 0.0,
+                median_relX=float(np.median(relx.max(0))) if n else 0.0,
+                max_relT=float((dT / np.abs(T_orc[reg])).max()) if n else 0.0,
+                max_rel_eps=float(rele[np.abs(eps_o) > 0].max()) if n and (np.abs(eps_o) > 0).any() else 0.0,
+                median_rel_eps=float(np.median(rele[np.abs(eps_o) > 0])) if n and (np.abs(eps_o) > 0).any() else 0.0)

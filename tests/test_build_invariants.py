@@ -1,0 +1,58 @@
+"""Properties of the built gfx950 code object (CPU only, no GPU).
+
+Every LDS access of the solve kernels must stay a direct LDS instruction: the matrix fill and the
+FP64 solve are lambdas forced inline with a single call site.  A build in which the compiler
+emitted the FP64 solve as a separate function (its LDS traffic then goes through generic
+addresses) faulted on the device, so the only non-kernel functions allowed in the code object
+are the single-lane helper routines declared __noinline__ on purpose."""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "sph-exa-nnet_amd", "lib", "libnnet_mi355x.so")
+LLVM = "/opt/rocm/llvm/bin"
+ALLOWED_HELPERS = ("special_triple_alpha", "special_c12c12", "special_c12o16", "special_o16o16", "special_c12ag",
+                   "special_o16ag", "helmholtz_cv_dpdt", "electron_interpolate")
+
+
+def _functions(tmp_path):
+    for tool in ("llvm-objcopy", "clang-offload-bundler", "llvm-readelf"):
+        if not os.path.exists(os.path.join(LLVM, tool)):
+            pytest.skip(f"{tool} not available")
+    if not os.path.exists(LIB):
+        pytest.skip("libnnet_mi355x.so not built")
+    fb, co = str(tmp_path / "fb.bin"), str(tmp_path / "co.o")
+    subprocess.run([os.path.join(LLVM, "llvm-objcopy"), f"--dump-section=.hip_fatbin={fb}", LIB, str(tmp_path / "x")],
+                   check=True)
+    subprocess.run([os.path.join(LLVM, "clang-offload-bundler"), "--unbundle", "--type=o", f"--input={fb}",
+                    "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={co}"], check=True)
+    out = subprocess.run([os.path.join(LLVM, "llvm-readelf"), "-sW", co], check=True, capture_output=True,
+                         text=True).stdout
+    funcs = {}
+    for line in out.splitlines():
+        f = line.split()
+        if len(f) >= 8 and f[3] == "FUNC":
+            funcs[f[7]] = (int(f[2]), f[4])
+    return funcs
+
+
+def test_only_kernels_and_declared_helpers(tmp_path):
+    funcs = _functions(tmp_path)
+    kernels = [n for n, (_, bind) in funcs.items() if bind == "GLOBAL"]
+    assert any("nnet_solve_kernel" in k for k in kernels)
+    others = [n for n, (_, bind) in funcs.items() if bind != "GLOBAL"]
+    unexpected = [n for n in others if not any(h in n for h in ALLOWED_HELPERS)]
+    assert not unexpected, f"functions outlined from the kernels: {unexpected}"
+
+
+def test_every_precision_and_network_is_built(tmp_path):
+    names = set(_functions(tmp_path))
+    for net in (14, 86, 87):
+        for eos in (0, 1):
+            for mix in (0, 1):
+                pat = re.compile(rf"nnet_solve_kernelILi{net}ELi{eos}ELb{mix}E")
+                assert any(pat.search(n) for n in names), (net, eos, mix)
