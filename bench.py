@@ -129,6 +129,9 @@ def main():
         """Warmup + timed hydro steps of one precision from the seeded initial state."""
         cfg, st = device_config(torch, args.config, n)
         prec = nm.PRECISION_FP64 if precision == "fp64" else nm.PRECISION_MIXED
+        # each step dispatches the particles in descending order of the previous step's Newton
+        # iterations (the newton output array doubles as the cost hint)
+        st["cost_hint"] = st["newton"]
         batch = ctx.make_batch(cfg["net"], st, eos=cfg["eos"], mu=cfg["mu"], Z=cfg["Z"], precision=prec)
         hdt = cfg["hydro_dt"]
         print(f"# {cfg['label']}: n={n} precision={precision} warmup={args.warmup} steps={args.steps}",

@@ -108,6 +108,11 @@ typedef struct nnet_batch {
     uint32_t* fp64_solves; /* device, n, optional: mixed precision only, Newton iterations solved
                               in FP64: the first whose FP32 factors did not refine to FP64
                               accuracy, and all later ones of that particle in this call */
+    const uint32_t* cost_hint; /* device, n, optional: expected relative cost per particle (e.g. the
+                              previous call's `newton` counts; the same array may be passed as
+                              `newton`).  Particles are then dispatched in descending buckets of
+                              the hint, so the longest integrations start first.  Results do not
+                              depend on it. */
 } nnet_batch;
 
 /* computeNuclearReactions for nb batches (different networks may share one call).

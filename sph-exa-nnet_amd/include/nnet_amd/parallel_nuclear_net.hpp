@@ -284,6 +284,7 @@ struct Options
     uint32_t* substeps  = nullptr; // device (optional)
     uint32_t* newton    = nullptr; // device (optional)
     uint32_t* fp64_solves = nullptr; // device (optional, mixed precision)
+    const uint32_t* cost_hint = nullptr; // device (optional): expected cost per particle, e.g. newton of the last call
 };
 
 template<bool use_gpu, class Data, typename Float, class List, int KIND, class Eos, class nseFunction = void*>
@@ -321,6 +322,7 @@ void computeNuclearReactions(Data& n, size_t firstIndex, size_t lastIndex, const
     b.substeps  = opt.substeps;
     b.newton    = opt.newton;
     b.fp64_solves = opt.fp64_solves;
+    b.cost_hint   = opt.cost_hint;
     if constexpr (std::is_same_v<Eos, eos::IdealGasFunctor>)
     {
         b.eos = NNET_EOS_IDEAL;

@@ -40,6 +40,7 @@ class Batch(C.Structure):
         ("substeps", C.c_void_p),
         ("newton", C.c_void_p),
         ("fp64_solves", C.c_void_p),
+        ("cost_hint", C.c_void_p),
     ]
 
 
@@ -120,7 +121,7 @@ class Context:
 
     def make_batch(self, net, state, eos=EOS_IDEAL, mu=10.0, Z=None, precision=PRECISION_FP64):
         """state: dict of torch device tensors rho, rho_m1, Y (d x n), temp, dt, status, substeps, newton
-        and (optional) fp64_solves."""
+        and (optional) fp64_solves and cost_hint."""
         b = Batch()
         b.network = self.network(net).value
         b.eos = eos
@@ -141,6 +142,7 @@ class Context:
         b.rho, b.rho_m1, b.temp, b.dt = ptr("rho"), ptr("rho_m1"), ptr("temp"), ptr("dt")
         b.status, b.substeps, b.newton = ptr("status"), ptr("substeps"), ptr("newton")
         b.fp64_solves = ptr("fp64_solves")
+        b.cost_hint = ptr("cost_hint")
         yp = (C.c_void_p * Y.shape[0])(*[Y[s].data_ptr() for s in range(Y.shape[0])])
         b.Y = C.cast(yp, C.POINTER(C.c_void_p))
         b._ykeep = yp
