@@ -116,6 +116,7 @@ struct BatchDev
     uint32_t*     substeps;
     uint32_t*     newton;
     uint32_t*     fp64_solves;
+    const uint32_t* order; // dispatch order of the particles (nullptr: index order)
     unsigned int* queue; // work-queue counter (zeroed by nnet_queue_reset before the launch)
     double*       Y[MAXSP]; // per-species abundance arrays (SoA)
 };
@@ -194,11 +195,12 @@ struct Lds
 // result layout of v_mfma_f64_16x16x4f64: register v of lane l of a tile holds row 16*I + (l>>4)
 // + 4v, column 16*J + (l&15).  Wave (wr, wc) of the WR x WR wave grid owns tiles (wr + WR*ti,
 // wc + WR*tj).  Elimination proceeds in blocks of four pivots (eigen.hpp:182-206 order per
-// element): the owners publish the block's four columns and four rows to LDS, every wave factors
-// the 4x4 diagonal block redundantly, derives its multipliers (A operand, rows) and normalised
-// pivot-row entries (B operand, columns) with the reference's 1e-40 skip test (a skipped
-// multiplier is zero, so its update adds zero), and applies the
-// rank-4 trailing update to its tiles with one MFMA per tile.  One barrier per block.
+// element): the owners publish the block's four columns and four rows to LDS; every lane factors
+// the 4x4 diagonal block; one lane per trailing row turns its four panel entries into the
+// multipliers (with the reference's 1e-40 skip test: a skipped multiplier is zero, so its update
+// adds zero) and one lane per trailing column its four pivot-row entries into the normalised
+// ones, in place; after a barrier every wave reads its A/B operands and applies the rank-4
+// trailing update to its tiles with one MFMA per tile.  Two barriers per block of four pivots.
 // ---------------------------------------------------------------------------------------------
 typedef double dx4 __attribute__((ext_vector_type(4)));
 typedef float  fx4 __attribute__((ext_vector_type(4)));
@@ -291,11 +293,14 @@ struct TileLU
                                                 int lr, int lc, int tid)
     {
         constexpr int p0 = 4 * STEP, Tp = p0 >> 4, nq = (N - p0) < 4 ? (N - p0) : 4;
-        const R       eps = (R)k::epsilonSystem;
-        const R*      P   = pan + (STEP & 1) * 8 * NPAD;
-        const R*      RT  = P + 4 * NPAD;
+        constexpr int NT = C::NT, HALF = NT / 2;
+        static_assert(HALF >= N - 3, "one lane per trailing row and per trailing column");
+        const R eps = (R)k::epsilonSystem;
+        R*      P   = pan + (STEP & 1) * 8 * NPAD;
+        R*      RT  = P + 4 * NPAD;
 
-        // 4x4 diagonal block (columns p0 .. p0+3 may include the right-hand side in the last block)
+        // 4x4 diagonal block (columns p0 .. p0+3 may include the right-hand side in the last
+        // block), factored in every lane
         R Dg[4][4], Un[4][4], Wt[4][4], rinv[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -340,19 +345,17 @@ struct TileLU
                 }
             }
 
-        // multipliers of this lane's rows (A operand: row arow(l&15) of the tile, pivot l>>4), negated
-        R Aop[TI];
-#pragma unroll
-        for (int ti = 0; ti < TI; ++ti)
+        if (tid < HALF)
         {
-            const int I = wr + WR * ti, i = 16 * I + TileT<R>::arow(lc);
-            Aop[ti]     = 0;
-            if (I >= Tp && i >= p0 + 4 && i < N)
+            // one lane per trailing row: its four multipliers, in place of its panel entries
+            const int i = p0 + 4 + tid;
+            if (i < N)
             {
-                R pr[4], Lq[4] = {0, 0, 0, 0};
+                R pr[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     pr[c] = P[i * 4 + c];
+                R Lq[4] = {0, 0, 0, 0};
 #pragma unroll
                 for (int q = 0; q < nq; ++q)
                 {
@@ -362,20 +365,20 @@ struct TileLU
                     for (int c = q + 1; c < nq; ++c)
                         pr[c] = fma(-w, Un[q][c], pr[c]);
                 }
-                const R mine = pick4(Lq, lr);
-                Aop[ti]      = -mine;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    P[i * 4 + q] = Lq[q];
                 if constexpr (STORE_L)
-                    if (wc == 0 && lr < nq) sL[lrow(i) + p0 + lr] = mine;
+#pragma unroll
+                    for (int q = 0; q < nq; ++q)
+                        sL[lrow(i) + p0 + q] = Lq[q];
             }
         }
-        // normalised pivot-row entries of this lane's columns (B operand: pivot l>>4, column l&15)
-        R Bop[TI];
-#pragma unroll
-        for (int tj = 0; tj < TI; ++tj)
+        else
         {
-            const int J = wc + WR * tj, j = 16 * J + lc;
-            Bop[tj]     = 0;
-            if (J >= Tp && j >= p0 + 4 && j <= N)
+            // one lane per trailing column: the normalised pivot-row entries, in place
+            const int j = p0 + 4 + (tid - HALF);
+            if (j <= N)
             {
                 R u[4];
 #pragma unroll
@@ -389,12 +392,31 @@ struct TileLU
                     for (int r = q + 1; r < nq; ++r)
                         u[r] = fma(-Wt[r][q], u[q], u[r]);
                 }
-                const R mine = lr < nq ? pick4(u, lr) : R(0);
-                Bop[tj]      = mine;
-                if (wr == 0 && lr < nq) sU[urow(p0 + lr) + j] = mine;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    RT[j * 4 + q] = q < nq ? u[q] : R(0);
+                    if (q < nq) sU[urow(p0 + q) + j] = u[q];
+                }
             }
         }
+        __syncthreads();
+
         // rank-4 trailing update, one MFMA per tile that holds trailing rows and columns
+        // (A operand: row arow(l&15) of the tile, pivot l>>4; B operand: pivot l>>4, column l&15)
+        R Aop[TI], Bop[TI];
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti)
+        {
+            const int i = 16 * (wr + WR * ti) + TileT<R>::arow(lc);
+            Aop[ti]     = (i >= p0 + 4 && i < N) ? -P[i * 4 + lr] : R(0);
+        }
+#pragma unroll
+        for (int tj = 0; tj < TI; ++tj)
+        {
+            const int j = 16 * (wc + WR * tj) + lc;
+            Bop[tj]     = (j >= p0 + 4 && j <= N) ? RT[j * 4 + lr] : R(0);
+        }
 #pragma unroll
         for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
@@ -525,8 +547,8 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
         __syncthreads();
         if (tid == 0) s_p = atomicAdd(bd.queue, 1u);
         __syncthreads();
-        const size_t p = s_p;
-        if (p >= bd.n) break;
+        if (s_p >= bd.n) break;
+        const size_t p = bd.order ? bd.order[s_p] : s_p;
 
         const double rho = bd.rho[p];
         double       T   = bd.temp[p];
@@ -592,11 +614,28 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             {
                 if (tid == NT - 1)
                 {
+                    // helmholtz.hpp:221-222: sum(Y) and sum(Y Z), each in index order; eight
+                    // abundances are loaded ahead of the two interleaved addition chains
                     double abar = 0., zbar = 0;
-                    for (int s = 0; s < ed.zdim; ++s)
+                    int    s    = 0;
+                    for (; s + 8 <= ed.zdim; s += 8)
+                    {
+                        double y[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            y[u] = sYb[s + u];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                        {
+                            abar += y[u];
+                            zbar += y[u] * ed.Z[s + u];
+                        }
+                    }
+                    for (; s < ed.zdim; ++s)
+                    {
                         abar += sYb[s];
-                    for (int s = 0; s < ed.zdim; ++s)
                         zbar += sYb[s] * ed.Z[s];
+                    }
                     double c0, d0;
                     helmholtz_cv_dpdt(ed.grid, ed.cells, abar, zbar, T_theta, rho_t, c0, d0);
                     sEos[0] = c0;
@@ -1224,6 +1263,64 @@ __global__ void nnet_queue_reset(unsigned int* q, int n)
     if (i < n) q[i] = 0u;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Cost-ordered dispatch: particles are handed to the persistent workgroups in descending buckets
+// of an expected cost (the caller's hint, e.g. the Newton iterations of the previous call), so the
+// long integrations start first and do not form a tail.  Bucket = bit length of the hint.
+// ---------------------------------------------------------------------------------------------
+constexpr int NBUCKET = 32;
+__device__ __forceinline__ int cost_bucket(uint32_t c)
+{
+    const int b = c == 0 ? 0 : 32 - __clz(c);
+    return b < NBUCKET ? b : NBUCKET - 1;
+}
+
+// bins[0 .. NBUCKET): counts; zeroed by nnet_queue_reset before the launch
+__global__ void __launch_bounds__(256) nnet_order_count(const uint32_t* __restrict__ cost, size_t n, unsigned* bins)
+{
+    __shared__ unsigned h[NBUCKET];
+    if (threadIdx.x < NBUCKET) h[threadIdx.x] = 0u;
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        atomicAdd(&h[cost_bucket(cost[i])], 1u);
+    __syncthreads();
+    if (threadIdx.x < NBUCKET && h[threadIdx.x]) atomicAdd(&bins[threadIdx.x], h[threadIdx.x]);
+}
+
+// bins[NBUCKET + b] = first position of bucket b (largest bucket first)
+__global__ void nnet_order_offsets(unsigned* bins)
+{
+    if (threadIdx.x == 0)
+    {
+        unsigned acc = 0;
+        for (int b = NBUCKET - 1; b >= 0; --b)
+        {
+            bins[NBUCKET + b] = acc;
+            acc += bins[b];
+        }
+    }
+}
+
+// each 256-particle tile reserves its bucket ranges with one atomic per bucket
+__global__ void __launch_bounds__(256) nnet_order_scatter(const uint32_t* __restrict__ cost, size_t n, unsigned* bins,
+                                                         uint32_t* __restrict__ order)
+{
+    __shared__ unsigned h[NBUCKET], base[NBUCKET];
+    for (size_t t0 = (size_t)blockIdx.x * blockDim.x; t0 < n; t0 += (size_t)gridDim.x * blockDim.x)
+    {
+        if (threadIdx.x < NBUCKET) h[threadIdx.x] = 0u;
+        __syncthreads();
+        const size_t i    = t0 + threadIdx.x;
+        const int    b    = i < n ? cost_bucket(cost[i]) : 0;
+        const unsigned r  = i < n ? atomicAdd(&h[b], 1u) : 0u;
+        __syncthreads();
+        if (threadIdx.x < NBUCKET && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&bins[NBUCKET + threadIdx.x], h[threadIdx.x]);
+        __syncthreads();
+        if (i < n) order[base[b] + r] = (uint32_t)i;
+        __syncthreads();
+    }
+}
+
 } // namespace nnet_amd
 
 // =============================================================================================
@@ -1244,6 +1341,11 @@ struct nnet_ctx
     unsigned int* d_queue = nullptr;
     int           n_queue = 0;
     int           n_cu    = 0;
+    // cost-ordered dispatch: per batch NBUCKET counts + NBUCKET offsets, and the particle order
+    unsigned int* d_bins   = nullptr;
+    uint32_t*     d_order  = nullptr;
+    size_t        n_order  = 0;
+    int           n_bins   = 0;
     std::vector<void*> owned;
 };
 
@@ -1623,6 +1725,8 @@ extern "C"
         if (c->d_grid) (void)hipFree(c->d_grid);
         if (c->d_cells) (void)hipFree(c->d_cells);
         if (c->d_queue) (void)hipFree(c->d_queue);
+        if (c->d_bins) (void)hipFree(c->d_bins);
+        if (c->d_order) (void)hipFree(c->d_order);
         for (void* p : c->owned)
             (void)hipFree(p);
         delete c;
@@ -1761,6 +1865,31 @@ extern "C"
             HIP_TRY(c, hipMalloc(&c->d_queue, sizeof(unsigned int) * std::max(nb, 8)));
             c->n_queue = std::max(nb, 8);
         }
+        // cost-ordered dispatch buffers: grow only when a call needs more than before
+        {
+            size_t need = 0;
+            for (int bi = 0; bi < nb; ++bi)
+                if (batches[bi].cost_hint) need += batches[bi].n;
+            if (need > c->n_order || (need > 0 && nb > c->n_bins))
+            {
+                HIP_TRY(c, hipStreamSynchronize(st));
+                if (need > c->n_order)
+                {
+                    if (c->d_order) HIP_TRY(c, hipFree(c->d_order));
+                    c->d_order = nullptr;
+                    HIP_TRY(c, hipMalloc(&c->d_order, sizeof(uint32_t) * need));
+                    c->n_order = need;
+                }
+                if (nb > c->n_bins)
+                {
+                    if (c->d_bins) HIP_TRY(c, hipFree(c->d_bins));
+                    c->d_bins = nullptr;
+                    HIP_TRY(c, hipMalloc(&c->d_bins, sizeof(unsigned int) * 2 * NBUCKET * std::max(nb, 8)));
+                    c->n_bins = std::max(nb, 8);
+                }
+            }
+        }
+        size_t order_off = 0;
         for (int bi = 0; bi < nb; ++bi)
         {
             const nnet_batch& b = batches[bi];
@@ -1799,6 +1928,19 @@ extern "C"
             const NetDev& nd = b.network->dev;
             bd.queue         = c->d_queue + bi;
             hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, st, bd.queue, 1);
+            bd.order = nullptr;
+            if (b.cost_hint)
+            {
+                unsigned int* bins  = c->d_bins + (size_t)2 * NBUCKET * bi;
+                uint32_t*     order = c->d_order + order_off;
+                order_off += b.n;
+                const unsigned blocks = (unsigned)std::min<size_t>((b.n + 255) / 256, 2048);
+                hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, st, bins, 2 * NBUCKET);
+                hipLaunchKernelGGL(nnet_order_count, dim3(blocks), dim3(256), 0, st, b.cost_hint, b.n, bins);
+                hipLaunchKernelGGL(nnet_order_offsets, dim3(1), dim3(64), 0, st, bins);
+                hipLaunchKernelGGL(nnet_order_scatter, dim3(blocks), dim3(256), 0, st, b.cost_hint, b.n, bins, order);
+                bd.order = order;
+            }
             const int kind = b.network->kind;
             auto      launch = [&](auto kern, int nt) -> int
             {
