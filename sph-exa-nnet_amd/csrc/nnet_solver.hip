@@ -624,6 +624,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
 #pragma unroll
                         for (int u = 0; u < 8; ++u)
                             y[u] = sYb[s + u];
+                        __builtin_amdgcn_sched_barrier(0); // keep the eight loads in flight together
 #pragma unroll
                         for (int u = 0; u < 8; ++u)
                         {
@@ -859,19 +860,28 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                 const int k1 = nzp16[e + 1];
                 for (; kk + 8 <= k1; kk += 8)
                 {
-                    double t[8];
-                    bool   ok[8];
+                    // stage 1: the eight entry words; stage 2: their factors; then the additions
+                    // (scheduling barriers keep each stage's loads in flight together)
+                    uint32_t w[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        w[u] = nzent[kk + u];
+                    __builtin_amdgcn_sched_barrier(0);
+                    double tr[8], okv[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
                     {
-                        const uint32_t w = nzent[kk + u];
-                        const int      r = w & 0xfff;
-                        ok[u]            = sJok[r] != 0.;
-                        t[u]             = sTr[r * MAXQ + ((w >> 12) & 0xf)] * (double)(int8_t)(w >> 16);
+                        const int r = w[u] & 0xfff;
+                        okv[u]      = sJok[r];
+                        tr[u]       = sTr[r * MAXQ + ((w[u] >> 12) & 0xf)];
                     }
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
-                        J = ok[u] ? J + t[u] : J;
+                    {
+                        const double t = tr[u] * (double)(int8_t)(w[u] >> 16);
+                        J              = okv[u] != 0. ? J + t : J;
+                    }
                 }
                 for (; kk < k1; ++kk)
                 {
@@ -892,21 +902,26 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                     const int e1 = spp16[s + 1];
                     for (; e + 8 <= e1; e += 8)
                     {
+                        uint32_t w[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            w[u] = spent[e + u];
+                        __builtin_amdgcn_sched_barrier(0);
                         double x[8], y[8];
 #pragma unroll
                         for (int u = 0; u < 8; ++u)
                         {
-                            const uint32_t w = spent[e + u];
-                            const int      r = w & 0xffff;
-                            const double   c = (double)(int8_t)(w >> 16);
-                            x[u]             = sVal[r] * c;
-                            y[u]             = sDval[r] * c;
+                            const int r = w[u] & 0xffff;
+                            x[u]        = sVal[r];
+                            y[u]        = sDval[r];
                         }
+                        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                         for (int u = 0; u < 8; ++u)
                         {
-                            a = a + x[u];
-                            b = b + y[u];
+                            const double c = (double)(int8_t)(w[u] >> 16);
+                            a              = a + x[u] * c;
+                            b              = b + y[u] * c;
                         }
                     }
                     for (; e < e1; ++e)
@@ -932,17 +947,24 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                 const int e1   = rowp16[tid + 1];
                 for (; e + 8 <= e1; e += 8)
                 {
-                    double x[8];
+                    uint32_t w[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        w[u] = rowent[e + u];
+                    __builtin_amdgcn_sched_barrier(0);
+                    double jv[8], yb[8], y0[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
                     {
-                        const uint32_t w = rowent[e + u];
-                        const int      b = w >> 16;
-                        x[u]             = sJv[w & 0xffff] * (sYb[b] - sY[b]);
+                        const int b = w[u] >> 16;
+                        jv[u]       = sJv[w[u] & 0xffff];
+                        yb[u]       = sYb[b];
+                        y0[u]       = sY[b];
                     }
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
-                        corr = corr + x[u];
+                        corr = corr + jv[u] * (yb[u] - y0[u]);
                 }
                 for (; e < e1; ++e)
                 {
