@@ -66,7 +66,7 @@ __device__ __forceinline__ double h5(const double* fi, double w0t, double w1t, d
 
 // cv and dP/dT of the Helmholtz EOS for abar_ = sum(Y), zbar_ = sum(Y Z) (helmholtz.hpp:217-225,
 // helmholtz.cpp:340-955).
-__device__ __noinline__ void helmholtz_cv_dpdt(const HelmGrid* __restrict__ g, const double* __restrict__ cells, double abar_,
+__device__ __forceinline__ void helmholtz_cv_dpdt(const HelmGrid* __restrict__ g, const double* __restrict__ cells, double abar_,
                                   double zbar_, double temp, double rho, double& cv_out, double& dpdT_out)
 {
     using namespace helmc;

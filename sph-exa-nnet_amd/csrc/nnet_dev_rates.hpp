@@ -108,7 +108,7 @@ struct DerivPow
 };
 
 // Group 0: triple alpha, its reverse, and derivatives (net14.cpp:218-243, 366-423).
-__device__ __noinline__ void special_triple_alpha(double T, double& r, double& rl, double& dr, double& drl)
+__device__ __forceinline__ void special_triple_alpha(double T, double& r, double& rl, double& dr, double& drl)
 {
     {
         const RatePow p(T);
@@ -161,7 +161,7 @@ __device__ __noinline__ void special_triple_alpha(double T, double& r, double& r
 }
 
 // Group 1: 12C + 12C (net14.cpp:248-259, 428-443).
-__device__ __noinline__ void special_c12c12(double T, double& r, double& dr)
+__device__ __forceinline__ void special_c12c12(double T, double& r, double& dr)
 {
     {
         const RatePow p(T);
@@ -180,7 +180,7 @@ __device__ __noinline__ void special_c12c12(double T, double& r, double& dr)
 }
 
 // Group 2: 12C + 16O (net14.cpp:264-281, 448-473).
-__device__ __noinline__ void special_c12o16(double T, double& r, double& dr)
+__device__ __forceinline__ void special_c12o16(double T, double& r, double& dr)
 {
     r  = 0;
     dr = 0;
@@ -216,7 +216,7 @@ __device__ __noinline__ void special_c12o16(double T, double& r, double& dr)
 }
 
 // Group 3: 16O + 16O (net14.cpp:286-293, 478-487).
-__device__ __noinline__ void special_o16o16(double T, double& r, double& dr)
+__device__ __forceinline__ void special_o16o16(double T, double& r, double& dr)
 {
     {
         const RatePow p(T);
@@ -229,7 +229,7 @@ __device__ __noinline__ void special_o16o16(double T, double& r, double& dr)
 }
 
 // Group 4: 12C(a,g)16O, reverse, derivatives (net14.cpp:298-314, 491-538).
-__device__ __noinline__ void special_c12ag(double T, double& r, double& rl, double& dr, double& drl)
+__device__ __forceinline__ void special_c12ag(double T, double& r, double& rl, double& dr, double& drl)
 {
     {
         const RatePow p(T);
@@ -269,7 +269,7 @@ __device__ __noinline__ void special_c12ag(double T, double& r, double& rl, doub
 }
 
 // Group 5: 16O(a,g)20Ne, reverse, derivatives (net14.cpp:319-334, 519-566).
-__device__ __noinline__ void special_o16ag(double T, double& r, double& rl, double& dr, double& drl)
+__device__ __forceinline__ void special_o16ag(double T, double& r, double& rl, double& dr, double& drl)
 {
     {
         const RatePow p(T);
@@ -314,7 +314,7 @@ __device__ __forceinline__ double coulomb_weak(double x)
 }
 
 // The electron-capture table interpolation of net87 (net87.cpp:102-141).
-__device__ __noinline__ void electron_interpolate(const double* logT, const double* logR, const double* tab, double temp,
+__device__ __forceinline__ void electron_interpolate(const double* logT, const double* logR, const double* tab, double temp,
                                      double rhoElec, double* rate)
 {
     const int nT = 41, nR = 51, nC = 12;

@@ -230,6 +230,14 @@ struct TileT<double>
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    // 1/d from the hardware estimate and two Newton steps (the pivot reciprocal is on the
+    // factorization's critical path; the IEEE division sequence is twice as long)
+    static __device__ __forceinline__ double recip(double d)
+    {
+        double r = __builtin_amdgcn_rcp(d);
+        r        = fma(r, fma(-d, r, 1.), r);
+        return fma(r, fma(-d, r, 1.), r);
+    }
 };
 template<>
 struct TileT<float>
@@ -241,6 +249,11 @@ struct TileT<float>
     static __device__ __forceinline__ V   mfma(float a, float b, V c)
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ float recip(float d)
+    {
+        const float r = __builtin_amdgcn_rcpf(d);
+        return fmaf(r, fmaf(-d, r, 1.f), r);
     }
 };
 
@@ -314,7 +327,7 @@ struct TileLU
 #pragma unroll
         for (int q = 0; q < nq; ++q)
         {
-            rinv[q] = R(1) / Dg[q][q];
+            rinv[q] = TileT<R>::recip(Dg[q][q]);
 #pragma unroll
             for (int c = q + 1; c < 4; ++c)
                 Un[q][c] = Dg[q][c] * rinv[q];

@@ -8,7 +8,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libnnet_mi355x.so")
+# NNET_MI355X_LIB selects another build of the library (e.g. to compare two builds on one GPU)
+LIB_PATH = os.environ.get("NNET_MI355X_LIB") or os.path.join(HERE, "lib", "libnnet_mi355x.so")
 
 RATES = {14: 14, 86: 86, 87: 87}
 EOS_IDEAL, EOS_HELMHOLTZ = 0, 1

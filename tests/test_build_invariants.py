@@ -1,10 +1,11 @@
 """Properties of the built gfx950 code object (CPU only, no GPU).
 
 Every LDS access of the solve kernels must stay a direct LDS instruction: the matrix fill and the
-FP64 solve are lambdas forced inline with a single call site.  A build in which the compiler
-emitted the FP64 solve as a separate function (its LDS traffic then goes through generic
-addresses) faulted on the device, so the only non-kernel functions allowed in the code object
-are the single-lane helper routines declared __noinline__ on purpose."""
+FP64 solve are lambdas forced inline with a single call site, and the single-lane rate/EOS
+routines are forced inline too.  A build in which the compiler emitted the FP64 solve as a
+separate function (its LDS traffic then goes through generic addresses) faulted on the device, so
+no function other than the kernels may appear in the code object; the helper names stay allowed
+should one of them be made a call again on purpose."""
 import os
 import re
 import shutil
