@@ -31,12 +31,14 @@ import workloads  # noqa: E402
 TABLE = os.path.join(ROOT, "sph-exa-nnet_amd", "data", "helm_table.bin")
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix)
 
-# HBM bytes per particle-step of the solve kernel on the net86 C3 distribution, from rocprofv3
+# HBM bytes per particle-step of the solve kernels on the net86 C3 distribution, from rocprofv3
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of `bench.py --steps 1 --warmup 0 --no-cpu` (one
-# dispatch each), scaled by the same counters on a plain 512 MiB device copy
+# dispatch per precision each), scaled by the same counters on a plain 512 MiB device copy
 # (`bench.py --copy-calibration 512`: FETCH_SIZE reads 0.5x and WRITE_SIZE 1.0x of the copied
-# bytes).  FP64 kernel: 130,068,949 KiB fetched + 142,210,817 KiB written per 2,097,152 particles.
-TRAFFIC_BYTES_PER_PARTICLE_STEP = {(3, "fp64"): (130068949 + 142210817) * 1024 / 2097152}
+# bytes).  FP64: 2 x 9,567,955 KiB fetched + 25,542,400 KiB written; mixed: 2 x 60,692,647 KiB
+# fetched + 82,258,290 KiB written, per 2,097,152 particles.
+TRAFFIC_BYTES_PER_PARTICLE_STEP = {(3, "fp64"): (2 * 9567955.06 + 25542399.59) * 1024 / 2097152,
+                                   (3, "mixed"): (2 * 60692647.44 + 82258289.88) * 1024 / 2097152}
 
 
 def flops_per_iter(net):
