@@ -49,23 +49,7 @@ def flops_per_iter(net):
 
 def device_config(torch, cid, n):
     """Upload the seeded particle set of configuration cid (Y built on the device)."""
-    cfg = workloads.config(cid, n=n, seed=workloads.SEED0 + cid)
-    dev = "cuda:0"
-    st = {
-        "rho": torch.tensor(cfg["rho"], dtype=torch.float64, device=dev),
-        "rho_m1": torch.tensor(cfg["rho"], dtype=torch.float64, device=dev),
-        "temp": torch.tensor(cfg["temp"], dtype=torch.float64, device=dev),
-        "dt": torch.tensor(cfg["dt"], dtype=torch.float64, device=dev),
-        "status": torch.zeros(n, dtype=torch.uint8, device=dev),
-        "substeps": torch.zeros(n, dtype=torch.int32, device=dev),
-        "newton": torch.zeros(n, dtype=torch.int32, device=dev),
-        "fp64_solves": torch.zeros(n, dtype=torch.int32, device=dev),
-    }
-    Y = torch.empty(cfg["Y"].shape, dtype=torch.float64, device=dev)
-    for s in range(cfg["Y"].shape[0]):
-        Y[s].copy_(torch.from_numpy(cfg["Y"][s]))
-    st["Y"] = Y
-    return cfg, st
+    return device_config_from(torch, workloads.config(cid, n=n, seed=workloads.SEED0 + cid))
 
 
 def cpu_baseline(cid, sample, steps_hint):
@@ -87,12 +71,98 @@ def cpu_baseline(cid, sample, steps_hint):
                       f"{el:.2f} s, statuses {np.bincount(st, minlength=5).tolist()}"}
 
 
+def run_c5(args, torch, nm):
+    """Mixed-network batch (net14 C2 + net86 C3 + net87 C4 distributions, 2:2:1) in one call per
+    hydro step; `--steps` steps with per-step dt log-uniform in [1e-6, 1e-3] are captured once in
+    a HIP graph; the timed region is `--warmup` untimed replays then one timed replay of the loop
+    (a replay continues from the state the previous one left)."""
+    parts, dts = workloads.config5(n=args.n, steps=args.steps)
+    prec = nm.PRECISION_FP64 if args.precision == "fp64" else nm.PRECISION_MIXED
+    ctx = nm.Context(0, helm_table=TABLE)
+    sts = []
+    for c in parts:
+        _, st = device_config_from(torch, c)
+        st["cost_hint"] = st["newton"]
+        sts.append(st)
+    bs = [ctx.make_batch(c["net"], st, eos=c["eos"], mu=c["mu"], Z=c["Z"], precision=prec)
+          for c, st in zip(parts, sts)]
+    n = sum(c["rho"].size for c in parts)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        ctx.compute(bs, float(dts[0]), stream=stream.cuda_stream, sync=False)  # sizes the library's buffers
+    stream.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        for dt in dts:
+            ctx.compute(bs, float(dt), stream=stream.cuda_stream, sync=False)
+    print(f"# C5: n={n} ({[c['rho'].size for c in parts]}), {len(dts)} steps captured", file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        graph.replay()
+        torch.cuda.synchronize()
+        print(f"# warmup replay {w} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    graph.replay()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    hist = np.zeros(8, np.int64)  # substeps of the last step: 0, 1, 2-3, 4-7, ... >= 64
+    for st in sts:
+        s_ = st["substeps"].cpu().numpy().astype(np.int64)
+        b = np.where(s_ == 0, 0, np.minimum(7, np.floor(np.log2(np.maximum(s_, 1))).astype(int) + 1))
+        hist += np.bincount(b, minlength=8)
+    out = {
+        "metric": "net86 particle-steps/s, mixed vs FP64 on 1 GPU at <=1e-6 rel mass-frac error",
+        "value": n * len(dts) / wall,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": len(dts),
+        "warmup": args.warmup,
+        "ms_per_step": wall / len(dts) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "mixed-network C5 (net14 + net86 + net87, one call per step, HIP graph)",
+                   "particles": [int(c["rho"].size) for c in parts], "precision": args.precision,
+                   "hydro_dt_range": [float(dts.min()), float(dts.max())],
+                   "last_step_substeps_hist_bins": ["0", "1", "2-3", "4-7", "8-15", "16-31", "32-63", ">=64"],
+                   "last_step_substeps_hist": hist.tolist()},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def device_config_from(torch, cfg):
+    """Upload one configuration dict's particle set (Y built on the device)."""
+    n = cfg["rho"].size
+    dev = "cuda:0"
+    st = {
+        "rho": torch.tensor(cfg["rho"], dtype=torch.float64, device=dev),
+        "rho_m1": torch.tensor(cfg["rho"], dtype=torch.float64, device=dev),
+        "temp": torch.tensor(cfg["temp"], dtype=torch.float64, device=dev),
+        "dt": torch.tensor(cfg["dt"], dtype=torch.float64, device=dev),
+        "status": torch.zeros(n, dtype=torch.uint8, device=dev),
+        "substeps": torch.zeros(n, dtype=torch.int32, device=dev),
+        "newton": torch.zeros(n, dtype=torch.int32, device=dev),
+        "fp64_solves": torch.zeros(n, dtype=torch.int32, device=dev),
+    }
+    Y = torch.empty(cfg["Y"].shape, dtype=torch.float64, device=dev)
+    for s in range(cfg["Y"].shape[0]):
+        Y[s].copy_(torch.from_numpy(cfg["Y"][s]))
+    st["Y"] = Y
+    return cfg, st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=3, choices=[2, 3, 4])
+    ap.add_argument("--config", type=int, default=3, choices=[2, 3, 4, 5],
+                    help="3 (default): net86 C3; 5: mixed-network batch, hydro loop captured in a HIP graph")
     ap.add_argument("--n", type=int, default=None, help="particles (default: the configuration's size)")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
                     help="both: the FP64 path, then the mixed path, each from the same initial state; "
@@ -123,6 +193,9 @@ def main():
     import torch
     import nnet_mi355x as nm
 
+    if args.config == 5:
+        run_c5(args, torch, nm)
+        return
     n = args.n or {2: 1048576, 3: 2097152, 4: 2097152}[args.config]
     ctx = nm.Context(0, helm_table=TABLE)
     stream = torch.cuda.current_stream()

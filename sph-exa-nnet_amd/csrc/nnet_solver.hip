@@ -1381,6 +1381,12 @@ struct nnet_ctx
     uint32_t*     d_order  = nullptr;
     size_t        n_order  = 0;
     int           n_bins   = 0;
+    // batches of one call run concurrently on side streams forked from (and joined back into)
+    // the caller's stream with events, so one batch's tail overlaps the others' work
+    static constexpr int NSIDE = 4;
+    hipStream_t          side[NSIDE]    = {};
+    hipEvent_t           side_ev[NSIDE] = {};
+    hipEvent_t           fork_ev        = nullptr;
     std::vector<void*> owned;
 };
 
@@ -1748,6 +1754,18 @@ extern "C"
             delete c;
             return NNET_E_HIP;
         }
+        for (int k = 0; k < nnet_ctx::NSIDE; ++k)
+            if (hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&c->side_ev[k], hipEventDisableTiming) != hipSuccess)
+            {
+                nnet_ctx_destroy(c);
+                return NNET_E_HIP;
+            }
+        if (hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess)
+        {
+            nnet_ctx_destroy(c);
+            return NNET_E_HIP;
+        }
         *out = c;
         if (helm_table_path) return nnet_ctx_load_helmholtz(c, helm_table_path);
         return NNET_OK;
@@ -1762,6 +1780,12 @@ extern "C"
         if (c->d_queue) (void)hipFree(c->d_queue);
         if (c->d_bins) (void)hipFree(c->d_bins);
         if (c->d_order) (void)hipFree(c->d_order);
+        for (int k = 0; k < nnet_ctx::NSIDE; ++k)
+        {
+            if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+            if (c->side_ev[k]) (void)hipEventDestroy(c->side_ev[k]);
+        }
+        if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
         for (void* p : c->owned)
             (void)hipFree(p);
         delete c;
@@ -1924,29 +1948,46 @@ extern "C"
                 }
             }
         }
-        size_t order_off = 0;
+        // validate every batch before anything is enqueued
+        int n_live = 0;
         for (int bi = 0; bi < nb; ++bi)
         {
             const nnet_batch& b = batches[bi];
             if (!b.network || b.network->ctx != c) return fail(c, NNET_E_ARG, "batch without a network of this context");
             if (b.n == 0) continue;
+            ++n_live;
             if (!b.rho || !b.Y || !b.temp || !b.dt || (use_drhodt && !b.rho_m1))
                 return fail(c, NNET_E_ARG, "null particle array");
             if (b.precision != NNET_PRECISION_FP64 && b.precision != NNET_PRECISION_MIXED)
                 return fail(c, NNET_E_ARG, "unknown precision");
-            const int d = b.network->d;
-            for (int s = 0; s < d; ++s)
-                if (!b.Y[s]) return fail(c, NNET_E_ARG, "null abundance array");
-            EosDev ed{};
-            ed.kind = b.eos;
-            ed.mu   = b.mu;
+            for (int z = 0; z < b.network->d; ++z)
+                if (!b.Y[z]) return fail(c, NNET_E_ARG, "null abundance array");
             if (b.eos == NNET_EOS_HELMHOLTZ)
             {
                 if (!c->helm_ok) return fail(c, NNET_E_STATE, "no Helmholtz table loaded");
-                if (!b.Z || b.z_dim <= 0 || b.z_dim > d) return fail(c, NNET_E_ARG, "Helmholtz Z/z_dim invalid");
+                if (!b.Z || b.z_dim <= 0 || b.z_dim > b.network->d) return fail(c, NNET_E_ARG, "Helmholtz Z/z_dim invalid");
             }
             else if (b.eos != NNET_EOS_IDEAL)
                 return fail(c, NNET_E_ARG, "unknown EOS");
+        }
+        const bool fork = n_live > 1;
+        if (fork)
+        {
+            HIP_TRY(c, hipEventRecord(c->fork_ev, st));
+            for (int k = 0; k < std::min(n_live, (int)nnet_ctx::NSIDE); ++k)
+                HIP_TRY(c, hipStreamWaitEvent(c->side[k], c->fork_ev, 0));
+        }
+        size_t order_off = 0;
+        int    live      = 0;
+        for (int bi = 0; bi < nb; ++bi)
+        {
+            const nnet_batch& b = batches[bi];
+            if (b.n == 0) continue;
+            const hipStream_t bst = fork ? c->side[live++ % nnet_ctx::NSIDE] : st;
+            const int         d   = b.network->d;
+            EosDev            ed{};
+            ed.kind = b.eos;
+            ed.mu   = b.mu;
             if (b.eos == NNET_EOS_HELMHOLTZ)
             {
                 for (int z = 0; z < b.z_dim; ++z)
@@ -1962,7 +2003,7 @@ extern "C"
                 bd.Y[z] = b.Y[z];
             const NetDev& nd = b.network->dev;
             bd.queue         = c->d_queue + bi;
-            hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, st, bd.queue, 1);
+            hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, bst, bd.queue, 1);
             bd.order = nullptr;
             if (b.cost_hint)
             {
@@ -1970,10 +2011,10 @@ extern "C"
                 uint32_t*     order = c->d_order + order_off;
                 order_off += b.n;
                 const unsigned blocks = (unsigned)std::min<size_t>((b.n + 255) / 256, 2048);
-                hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, st, bins, 2 * NBUCKET);
-                hipLaunchKernelGGL(nnet_order_count, dim3(blocks), dim3(256), 0, st, b.cost_hint, b.n, bins);
-                hipLaunchKernelGGL(nnet_order_offsets, dim3(1), dim3(64), 0, st, bins);
-                hipLaunchKernelGGL(nnet_order_scatter, dim3(blocks), dim3(256), 0, st, b.cost_hint, b.n, bins, order);
+                hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, bst, bins, 2 * NBUCKET);
+                hipLaunchKernelGGL(nnet_order_count, dim3(blocks), dim3(256), 0, bst, b.cost_hint, b.n, bins);
+                hipLaunchKernelGGL(nnet_order_offsets, dim3(1), dim3(64), 0, bst, bins);
+                hipLaunchKernelGGL(nnet_order_scatter, dim3(blocks), dim3(256), 0, bst, b.cost_hint, b.n, bins, order);
                 bd.order = order;
             }
             const int kind = b.network->kind;
@@ -1983,7 +2024,7 @@ extern "C"
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, 0) != hipSuccess || per_cu < 1)
                     per_cu = 1;
                 const size_t wgs = std::min<size_t>(b.n, (size_t)per_cu * c->n_cu);
-                hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(nt), 0, st, nd, ed, bd, sp);
+                hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(nt), 0, bst, nd, ed, bd, sp);
                 return 0;
             };
             const bool mix      = b.precision == NNET_PRECISION_MIXED;
@@ -2010,6 +2051,12 @@ extern "C"
                 dispatch(std::integral_constant<int, 87>{});
             HIP_TRY(c, hipGetLastError());
         }
+        if (fork)
+            for (int k = 0; k < std::min(n_live, (int)nnet_ctx::NSIDE); ++k)
+            {
+                HIP_TRY(c, hipEventRecord(c->side_ev[k], c->side[k]));
+                HIP_TRY(c, hipStreamWaitEvent(st, c->side_ev[k], 0));
+            }
         return NNET_OK;
     }
 

@@ -115,3 +115,17 @@ def config(cid, n=None, seed=None):
         return dict(net=87, eos=1, mu=0.0, Z=helmholtz_Z(87), rho=rho, temp=T, Y=Y, dt=np.full(n, INITIAL_DT),
                     hydro_dt=1e-5, steps=20, label="net87 Helmholtz C4")
     raise ValueError(cid)
+
+
+def config5(n=None, seed=None, steps=100):
+    """Mixed-network batch in one call: n14 net14 (C2 distribution) + n86 net86 (C3) + n87 net87 (C4)
+    particles with n14 = n86 = 2 n87 (n = n14 + n86 + n87, default 2,621,440), and the per-step
+    hydro dt, log-uniform in [1e-6, 1e-3], for `steps` steps.  Returns (list of three configs,
+    dts)."""
+    n = 2621440 if n is None else n
+    n87 = n // 5
+    n14 = n86 = (n - n87) // 2
+    base = SEED0 + 5 if seed is None else seed
+    parts = [config(2, n=n14, seed=base + 2), config(3, n=n86, seed=base + 3), config(4, n=n87, seed=base + 4)]
+    rng = np.random.default_rng(base)
+    return parts, _logu(rng, 1e-6, 1e-3, steps)
