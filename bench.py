@@ -135,6 +135,71 @@ def run_c5(args, torch, nm):
     ctx.close()
 
 
+def run_helm_eos(args, torch, nm):
+    """Post-step EOS refresh (computeHelmEOS -> nnet_helm_eos) over the particle set of
+    configuration `--config` (default C3: 2,097,152 net86 particles): `--warmup` untimed and
+    `--steps` timed launches on one stream, inputs resident in HBM.  Algorithmic HBM bytes per
+    particle: Y (d doubles), temp and rho read, u, cv, p, c, dP/dT written."""
+    cid = args.config
+    n = args.n or {2: 1048576, 3: 2097152, 4: 2097152}[cid]
+    cfg, st = device_config(torch, cid, n)
+    ctx = nm.Context(0, helm_table=TABLE)
+    stream = torch.cuda.current_stream()
+    out = {k: torch.empty(n, dtype=torch.float64, device="cuda:0") for k in ("u", "cv", "p", "c", "dpdT")}
+    launch = lambda: ctx.helm_eos(cfg["Z"], st["temp"], st["rho"], st["Y"], out, stream=stream.cuda_stream,
+                                  sync=False)
+    print(f"# helm-eos {cfg['label']}: n={n} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        launch()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        launch()
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_s = float(np.mean([x.elapsed_time(y) for x, y in ev])) / 1e3
+    d = cfg["Y"].shape[0]
+    bytes_per_particle = 8 * (d + 2 + 5)
+    achieved = bytes_per_particle * n / kernel_s
+    res = {
+        "metric": f"net{cfg['net']} Helmholtz EOS refresh (computeHelmEOS) particles/s on 1 GPU",
+        "value": n * args.steps / wall,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"EOS refresh over the {cfg['label']} particle set", "particles": n,
+                   "kernel_ms": kernel_s * 1e3, "algorithmic_bytes_per_particle": bytes_per_particle},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8e12, "traffic": None},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        o = Oracle()
+        o.helm_load(TABLE)
+        sample = min(n, 1 << 20)
+        c = workloads.config(cid, n=sample, seed=workloads.SEED0 + 100 + cid)
+        threads = os.cpu_count() or 1
+        t0 = time.perf_counter()
+        o.helm_eos_batch(c["Z"], c["temp"], c["rho"], c["Y"], nthreads=threads)
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": sample / el, "unit": "particles/s", "cores": threads, "kind": "port",
+                               "sample": f"{sample} particles of the same distribution (seed +100), {el:.2f} s"}
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
 def device_config_from(torch, cfg):
     """Upload one configuration dict's particle set (Y built on the device)."""
     n = cfg["rho"].size
@@ -167,6 +232,9 @@ def main():
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
                     help="both: the FP64 path, then the mixed path, each from the same initial state; "
                          "value is the mixed path's rate")
+    ap.add_argument("--workload", default="solve", choices=["solve", "helm-eos"],
+                    help="solve (default): computeNuclearReactions; helm-eos: the post-step EOS refresh "
+                         "computeHelmEOS over the same particle set")
     ap.add_argument("--cpu-sample", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -193,6 +261,9 @@ def main():
     import torch
     import nnet_mi355x as nm
 
+    if args.workload == "helm-eos":
+        run_helm_eos(args, torch, nm)
+        return
     if args.config == 5:
         run_c5(args, torch, nm)
         return
@@ -207,6 +278,9 @@ def main():
         # each step dispatches the particles in descending order of the previous step's Newton
         # iterations (the newton output array doubles as the cost hint)
         st["cost_hint"] = st["newton"]
+        # mixed: particles whose FP32 factors did not refine in the previous step go straight to
+        # the FP64 pass (the fp64_solves output doubles as the hint)
+        st["fp64_hint"] = st["fp64_solves"]
         batch = ctx.make_batch(cfg["net"], st, eos=cfg["eos"], mu=cfg["mu"], Z=cfg["Z"], precision=prec)
         hdt = cfg["hydro_dt"]
         print(f"# {cfg['label']}: n={n} precision={precision} warmup={args.warmup} steps={args.steps}",

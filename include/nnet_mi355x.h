@@ -45,8 +45,9 @@ extern "C" {
 
 /* arithmetic of the Newton linear solve */
 #define NNET_PRECISION_FP64 0  /* plain FP64 everywhere (parity path) */
-#define NNET_PRECISION_MIXED 1 /* FP32 LU on MFMA + FP64 iterative refinement (<= 3 sweeps,
-                                  FP64 re-solve of an iteration that does not converge) */
+#define NNET_PRECISION_MIXED 1 /* FP32 LU on MFMA + FP64 iterative refinement (<= 3 sweeps); a
+                                  particle with an iteration that does not converge is integrated
+                                  by the FP64 path instead */
 
 /* per-particle status written to nnet_batch.status */
 #define NNET_STATUS_OK 0      /* integrated to hydro_dt */
@@ -106,13 +107,19 @@ typedef struct nnet_batch {
     uint32_t* substeps;   /* device, n, optional: accepted substeps this call */
     uint32_t* newton;     /* device, n, optional: Newton iterations this call */
     uint32_t* fp64_solves; /* device, n, optional: mixed precision only, Newton iterations solved
-                              in FP64: the first whose FP32 factors did not refine to FP64
-                              accuracy, and all later ones of that particle in this call */
+                              in FP64.  A particle with an iteration whose FP32 factors do not
+                              refine to FP64 accuracy is integrated again from the start of the
+                              step by the FP64 pass: then fp64_solves = newton; else 0. */
     const uint32_t* cost_hint; /* device, n, optional: expected relative cost per particle (e.g. the
                               previous call's `newton` counts; the same array may be passed as
                               `newton`).  Particles are then dispatched in descending buckets of
                               the hint, so the longest integrations start first.  Results do not
                               depend on it. */
+    const uint32_t* fp64_hint; /* device, n, optional, mixed precision only: a particle with a
+                              nonzero entry skips the FP32 attempt and is integrated by the FP64
+                              pass (its result is then the FP64 path's).  Passing the previous
+                              call's fp64_solves (the same array may be passed as fp64_solves)
+                              keeps the particles whose FP32 factors did not refine on FP64. */
 } nnet_batch;
 
 /* computeNuclearReactions for nb batches (different networks may share one call).
@@ -132,6 +139,26 @@ int nnet_ctx_set_max_iterations(nnet_ctx* ctx, long max_iter);
 /* Blocking variant of nnet_compute (drop-in for the reference's synchronous launch). */
 int nnet_compute_sync(nnet_ctx* ctx, const nnet_batch* batches, int nb, double hydro_dt,
                       double previous_dt, int use_drhodt);
+
+/* Post-step EOS refresh for the hydro: replaces the device half of
+ *   template<bool use_gpu, class Data, class Vector>
+ *   void nnet::parallel::computeHelmEOS(Data& n, size_t firstIndex, size_t lastIndex, const Vector& Z)
+ *                                     (include/nnet/parallel/parallel_nuclear_net.hpp:191-261)
+ * i.e. cudaComputeHelmholtz (parallel/CUDA/parallel_nuclear_net.cu:185-226), with the semantics
+ * of the CPU path computeHelmholtzImpl (parallel_nuclear_net_impl.cpp:116-141): for each particle
+ * i in [0, n), abar_ = sum_j Y[j][i] and zbar_ = sum_j Y[j][i] Z[j] (index order), and the
+ * Helmholtz EOS (helmholtz.cpp:340-965) at (temp[i], rho[i]) gives u (specific internal energy),
+ * cv, p, c (sound speed) and dP/dT.  Z: host array of `dimension` charges (1 <= dimension <= 128);
+ * Y: host array of `dimension` device pointers, each n abundances; temp, rho: device, n; each
+ * output (device, n) may be NULL to skip it.  Needs a loaded Helmholtz table.  Stream-ordered. */
+int nnet_helm_eos(nnet_ctx* ctx, size_t n, int dimension, const double* Z, const double* temp,
+                  const double* rho, const double* const* Y, double* u, double* cv, double* p,
+                  double* c, double* dpdT, void* stream);
+/* Blocking variant of nnet_helm_eos (the reference's GPU branch synchronises,
+ * parallel_nuclear_net.hpp:234). */
+int nnet_helm_eos_sync(nnet_ctx* ctx, size_t n, int dimension, const double* Z, const double* temp,
+                       const double* rho, const double* const* Y, double* u, double* cv, double* p,
+                       double* c, double* dpdT);
 
 #ifdef __cplusplus
 }

@@ -625,6 +625,33 @@ extern "C"
         return tr.len;
     }
 
+    // computeHelmholtzImpl (parallel_nuclear_net_impl.cpp:116-141) over particles [0, n): per
+    // particle abar_ = sum_j Y[j][i] (std::accumulate from 0., index order) and
+    // zbar_ = sum_j Y[j][i] Z[j] (eigen::dot, eigen.hpp:157-166, index order), then the Helmholtz
+    // EOS (helmholtz.cpp:340-965); writes u, cv, p, c and dP/dT.  Y: dim pointers to per-species
+    // arrays.  nthreads <= 0 keeps the OpenMP default.
+    void oracle_helm_eos_batch(size_t n, int dim, const double* Z, const double* temp, const double* rho,
+                               double** Y, double* u, double* cv, double* p, double* c, double* dpdT, int nthreads)
+    {
+        if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < n; ++i)
+        {
+            double abar = 0.;
+            for (int j = 0; j < dim; ++j)
+                abar += Y[j][i];
+            double zbar = 0.;
+            for (int j = 0; j < dim; ++j)
+                zbar += Y[j][i] * Z[j];
+            const EosOut r = helm::eos(g_helm, abar, zbar, temp[i], rho[i]);
+            u[i]           = r.u;
+            cv[i]          = r.cv;
+            p[i]           = r.p;
+            c[i]           = r.c;
+            dpdT[i]        = r.dpdT;
+        }
+    }
+
     // computeNuclearReactionsImpl (parallel_nuclear_net_impl.cpp:63-103) over particles [0, n).
     // Y is an array of d pointers to per-species arrays (SoA), as in the reference.
     // status / substeps / newton may be null.  nthreads <= 0 keeps the OpenMP default.

@@ -90,6 +90,22 @@ class _Lib:
                                 _p(np.ascontiguousarray(Yg, dtype=np.float64)), Tg, rho, drho_dt, dt, _p(M), _p(R))
         return rc, M.reshape(n, n, order="F"), R
 
+    def helm_eos_batch(self, Z, temp, rho, Y, nthreads=0):
+        """computeHelmholtzImpl over the columns of Y (dim x n, C-contiguous rows): returns the
+        arrays u, cv, p, c, dpdT."""
+        f = self.fn("helm_eos_batch")
+        f.argtypes = [C.c_size_t, C.c_int, D, D, D, C.POINTER(D), D, D, D, D, D, C.c_int]
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        dim, n = Y.shape
+        Zc = np.ascontiguousarray(Z, dtype=np.float64)
+        assert Zc.shape == (dim,)
+        temp = np.ascontiguousarray(temp, dtype=np.float64)
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        rows = (D * dim)(*[Y[s].ctypes.data_as(D) for s in range(dim)])
+        out = [np.zeros(n) for _ in range(5)]
+        f(n, dim, _p(Zc), _p(temp), _p(rho), rows, *[_p(o) for o in out], nthreads)
+        return tuple(out)
+
     def particle(self, net, Y, T, rho, dt_tot, dt, eos_kind=0, mu=10.0, Z=None, max_iter=10000, trace_cap=0,
                  drho_dt=0.0):
         """One particle through the substep solver.  Returns (Y, T, dt, stats, trace); stats = status,

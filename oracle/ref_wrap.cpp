@@ -3,6 +3,7 @@
 // Used only to pin the oracle restatement (bit-exact comparisons) and as the "reference" CPU
 // baseline.  Nothing of the reference is copied: this file only calls its functions.
 #include <cmath>
+#include <omp.h>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -215,6 +216,14 @@ extern "C"
         nnet::parallel::computeNuclearReactionsImpl(n, dim_for(net), rho, rho_m1, Y, temp, dt, hydro_dt, previous_dt,
                                                     list_for(net), rates_for(net), h.get(eos_kind), use_drhodt,
                                                     (void*)nullptr);
+    }
+
+    // The reference's OpenMP post-step EOS refresh (parallel_nuclear_net_impl.cpp:116-141).
+    void ref_helm_eos_batch(size_t n, int dim, const double* Z, const double* temp, const double* rho, double** Y,
+                            double* u, double* cv, double* p, double* c, double* dpdT, int nthreads)
+    {
+        if (nthreads > 0) omp_set_num_threads(nthreads);
+        nnet::parallel::computeHelmholtzImpl(n, dim, Z, temp, rho, Y, u, cv, p, c, dpdT);
     }
 #endif
 }

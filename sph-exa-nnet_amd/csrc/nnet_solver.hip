@@ -117,7 +117,15 @@ struct BatchDev
     uint32_t*     newton;
     uint32_t*     fp64_solves;
     const uint32_t* order; // dispatch order of the particles (nullptr: index order)
-    unsigned int* queue; // work-queue counter (zeroed by nnet_queue_reset before the launch)
+    // positions [qrange[0], qrange[1]) of `order` form this launch's work (nullptr: [0, n)); the
+    // bounds are device values written by earlier kernels of the same stream
+    const unsigned int* qrange;
+    unsigned int*       queue; // work-queue counter (zeroed by nnet_queue_reset before the launch)
+    // mixed precision: the mixed pass appends the particles whose FP32 refinement failed to
+    // redo[0 .. *redo_count); an FP64 pass then integrates them from the start of the step
+    uint32_t*     redo;
+    unsigned int* redo_count;
+    int           fp64_pass; // FP64 pass of a mixed batch: fp64_solves = newton
     double*       Y[MAXSP]; // per-species abundance arrays (SoA)
 };
 
@@ -555,13 +563,14 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
     const uint16_t* rowp16 = reinterpret_cast<const uint16_t*>(simg) + nd.oRowPtr16;
     const uint32_t* rowent = simg + nd.oRowEnt;
 
+    const size_t q0 = bd.qrange ? (size_t)bd.qrange[0] : 0, q1 = bd.qrange ? (size_t)bd.qrange[1] : bd.n;
     for (;;)
     {
         __syncthreads();
         if (tid == 0) s_p = atomicAdd(bd.queue, 1u);
         __syncthreads();
-        if (s_p >= bd.n) break;
-        const size_t p = bd.order ? bd.order[s_p] : s_p;
+        if (q0 + s_p >= q1) break;
+        const size_t p = bd.order ? bd.order[q0 + s_p] : s_p;
 
         const double rho = bd.rho[p];
         double       T   = bd.temp[p];
@@ -592,8 +601,8 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
         double       T_buf   = __builtin_nan("");
         double       att_dt = 0, att_Tb = 0;
         long         total_iter = 0;
-        uint32_t     n_sub = 0, n_newton = 0, n_fallback = 0;
-        bool         fp64_only = false; // mixed path: FP32 refinement failed once for this particle
+        uint32_t     n_sub = 0, n_newton = 0;
+        bool         abandon = false; // mixed pass: FP32 refinement failed, left to the FP64 redo pass
         uint8_t      status = NNET_STATUS_OK;
 
         for (int i = 1;; ++i)
@@ -1081,119 +1090,110 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             };
 
             // the FP64 solve has a single (inlined) call site, so that every LDS access of the
-            // kernel stays a direct LDS instruction
-            bool use_fp64 = !MIX || fp64_only;
-            if constexpr (MIX)
+            // kernel stays a direct LDS instruction; the mixed kernel does not contain it
+            if constexpr (!MIX)
+                solve_fp64();
+            else
             {
-                if (!fp64_only)
+                // ---- mixed precision: FP32 tile LU + FP64 iterative refinement; a particle
+                //      with an iteration whose refinement does not converge is abandoned here
+                //      and integrated again, from the start of the step, by the FP64 pass ----
+                using LU32 = TileLU<NET, float, true>;
+                float* sU32   = reinterpret_cast<float*>(sm);
+                float* sL32   = sU32 + L::NU;
+                float* sRi32  = sL32 + L::NL;
+                float* sPan32 = reinterpret_cast<float*>(sPan);
                 {
-                    // ---- mixed precision: FP32 tile LU + FP64 iterative refinement; an iteration
-                    //      whose refinement does not converge is solved again in FP64, and so are the
-                    //      particle's remaining iterations in this call ----
-                    using LU32 = TileLU<NET, float, true>;
-                    float* sU32   = reinterpret_cast<float*>(sm);
-                    float* sL32   = sU32 + L::NU;
-                    float* sRi32  = sL32 + L::NL;
-                    float* sPan32 = reinterpret_cast<float*>(sPan);
+                    fx4 Mt[TI][TI];
+                    fill(Mt);
+                    __syncthreads(); // FP32 factors alias the head arrays only
+                    LU32::factor(Mt, sPan32, sU32, sL32, sRi32, wr, wc, lr, lc, tid);
+                }
+                if (tid < 64)
+                {
+                    float acc[NA];
+#pragma unroll
+                    for (int a = 0; a < NA; ++a)
                     {
-                        fx4 Mt[TI][TI];
-                        fill(Mt);
-                        __syncthreads(); // FP32 factors alias the head arrays only
-                        LU32::factor(Mt, sPan32, sU32, sL32, sRi32, wr, wc, lr, lc, tid);
+                        const int row = tid + 64 * a;
+                        acc[a]        = row < N ? sU32[LU32::urow(row) + N] : 0.f;
                     }
+                    LU32::back(sU32, acc, tid);
+#pragma unroll
+                    for (int a = 0; a < NA; ++a)
+                        if (tid + 64 * a < N) sX[tid + 64 * a] = (double)acc[a];
+                }
+                __syncthreads();
+                double* sRes = sPan; // the panel buffers are free after the LU
+                bool    conv = false;
+                for (int sweep = 0; sweep < k::mixedSweeps && !conv; ++sweep)
+                {
+                    // FP64 residual r = b - M x over the sparse pattern (row 0 is dense)
+                    for (int ri = tid; ri < N; ri += NT)
+                    {
+                        double r;
+                        if (ri == 0)
+                        {
+                            r = sRow0[N];
+                            for (int c = 0; c < N; ++c)
+                                r = fma(-sRow0[c], sX[c], r);
+                        }
+                        else
+                        {
+                            const int s = ri - 1;
+                            r           = entry(ri, N, 0.);
+                            r           = fma(-entry(ri, 0, 0.), sX[0], r);
+                            bool diag   = false;
+                            for (int e = rowp16[s]; e < rowp16[s + 1]; ++e)
+                            {
+                                const uint32_t w = rowent[e];
+                                const int      c = (int)(w >> 16) + 1;
+                                diag             = diag || c == ri;
+                                r                = fma(-entry(ri, c, sJv[w & 0xffff]), sX[c], r);
+                            }
+                            if (!diag) r -= sX[ri];
+                        }
+                        sRes[ri] = r;
+                    }
+                    __syncthreads();
+                    // correction d = (LU)^-1 r in FP32; converged when every |d_i| is below
+                    // mixedTol of its magnitude scale (|x_0| + T for the temperature, |x_i| + Y_i
+                    // for the abundances)
                     if (tid < 64)
                     {
                         float acc[NA];
-    #pragma unroll
+#pragma unroll
                         for (int a = 0; a < NA; ++a)
                         {
                             const int row = tid + 64 * a;
-                            acc[a]        = row < N ? sU32[LU32::urow(row) + N] : 0.f;
+                            acc[a]        = row < N ? (float)sRes[row] : 0.f;
                         }
+                        LU32::forward(sL32, sRi32, acc, tid);
                         LU32::back(sU32, acc, tid);
-    #pragma unroll
+                        bool ok = true;
+#pragma unroll
                         for (int a = 0; a < NA; ++a)
-                            if (tid + 64 * a < N) sX[tid + 64 * a] = (double)acc[a];
+                        {
+                            const int row = tid + 64 * a;
+                            if (row < N)
+                            {
+                                const double d     = (double)acc[a];
+                                const double xn    = sX[row] + d;
+                                const double scale = fabs(xn) + (row == 0 ? T : fabs(sY[row - 1])) + 1e-30;
+                                ok                 = ok && fabs(d) <= k::mixedTol * scale;
+                                sX[row]            = xn;
+                            }
+                        }
+                        const bool all = __all(ok);
+                        if (tid == 0) s_flag = all ? 1u : 0u;
                     }
                     __syncthreads();
-                    double* sRes = sPan; // the panel buffers are free after the LU
-                    bool    conv = false;
-                    for (int sweep = 0; sweep < k::mixedSweeps && !conv; ++sweep)
-                    {
-                        // FP64 residual r = b - M x over the sparse pattern (row 0 is dense)
-                        for (int ri = tid; ri < N; ri += NT)
-                        {
-                            double r;
-                            if (ri == 0)
-                            {
-                                r = sRow0[N];
-                                for (int c = 0; c < N; ++c)
-                                    r = fma(-sRow0[c], sX[c], r);
-                            }
-                            else
-                            {
-                                const int s = ri - 1;
-                                r           = entry(ri, N, 0.);
-                                r           = fma(-entry(ri, 0, 0.), sX[0], r);
-                                bool diag   = false;
-                                for (int e = rowp16[s]; e < rowp16[s + 1]; ++e)
-                                {
-                                    const uint32_t w = rowent[e];
-                                    const int      c = (int)(w >> 16) + 1;
-                                    diag             = diag || c == ri;
-                                    r                = fma(-entry(ri, c, sJv[w & 0xffff]), sX[c], r);
-                                }
-                                if (!diag) r -= sX[ri];
-                            }
-                            sRes[ri] = r;
-                        }
-                        __syncthreads();
-                        // correction d = (LU)^-1 r in FP32; converged when every |d_i| is below
-                        // mixedTol of its magnitude scale (|x_0| + T for the temperature, |x_i| + Y_i
-                        // for the abundances)
-                        if (tid < 64)
-                        {
-                            float acc[NA];
-    #pragma unroll
-                            for (int a = 0; a < NA; ++a)
-                            {
-                                const int row = tid + 64 * a;
-                                acc[a]        = row < N ? (float)sRes[row] : 0.f;
-                            }
-                            LU32::forward(sL32, sRi32, acc, tid);
-                            LU32::back(sU32, acc, tid);
-                            bool ok = true;
-    #pragma unroll
-                            for (int a = 0; a < NA; ++a)
-                            {
-                                const int row = tid + 64 * a;
-                                if (row < N)
-                                {
-                                    const double d     = (double)acc[a];
-                                    const double xn    = sX[row] + d;
-                                    const double scale = fabs(xn) + (row == 0 ? T : fabs(sY[row - 1])) + 1e-30;
-                                    ok                 = ok && fabs(d) <= k::mixedTol * scale;
-                                    sX[row]            = xn;
-                                }
-                            }
-                            const bool all = __all(ok);
-                            if (tid == 0) s_flag = all ? 1u : 0u;
-                        }
-                        __syncthreads();
-                        conv = s_flag != 0;
-                    }
-                    if (!conv)
-                    {
-                        use_fp64  = true;
-                        fp64_only = true;
-                    }
+                    conv = s_flag != 0;
                 }
+                abandon = !conv;
             }
-            if (use_fp64)
-            {
-                solve_fp64();
-                if constexpr (MIX) ++n_fallback;
-            }
+            if constexpr (MIX)
+                if (abandon) break;
             ++n_newton;
             ++total_iter;
 
@@ -1278,6 +1278,12 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             __syncthreads();
         }
         __syncthreads();
+        if constexpr (MIX)
+            if (abandon)
+            {
+                if (tid == 0) bd.redo[atomicAdd(bd.redo_count, 1u)] = (uint32_t)p;
+                continue;
+            }
         for (int s = tid; s < D; s += NT)
             bd.Y[s][p] = sY[s];
         if (tid == 0)
@@ -1287,9 +1293,67 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             if (bd.status) bd.status[p] = status;
             if (bd.substeps) bd.substeps[p] = n_sub;
             if (bd.newton) bd.newton[p] = n_newton;
-            if (bd.fp64_solves) bd.fp64_solves[p] = n_fallback;
+            if (bd.fp64_solves) bd.fp64_solves[p] = bd.fp64_pass ? n_newton : 0u;
         }
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Post-step EOS refresh (computeHelmholtzImpl, parallel_nuclear_net_impl.cpp:116-141): one lane
+// per particle.  The abundances are read species by species, each load coalesced across the
+// wavefront, eight at a time ahead of the two in-order sums (sum(Y), then sum(Y Z) as
+// eigen::dot); each lane then gathers its 288 + 128-byte cell blocks of the table.
+// ---------------------------------------------------------------------------------------------
+constexpr int HELM_EOS_MAX_DIM = 128;
+struct HelmEosArgs
+{
+    size_t        n;
+    int           dim;
+    const double* temp;
+    const double* rho;
+    double*       u;
+    double*       cv;
+    double*       p;
+    double*       c;
+    double*       dpdT;
+    const double* Y[HELM_EOS_MAX_DIM];
+    double        Z[HELM_EOS_MAX_DIM];
+};
+
+__global__ void __launch_bounds__(256) nnet_helm_eos_kernel(HelmEosArgs a, const HelmGrid* __restrict__ g,
+                                                            const double* __restrict__ cells,
+                                                            const double* __restrict__ cells3)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double abar = 0., zbar = 0.;
+    int    s    = 0;
+    for (; s + 8 <= a.dim; s += 8)
+    {
+        double y[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            y[u] = a.Y[s + u][i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            abar += y[u];
+            zbar += y[u] * a.Z[s + u];
+        }
+    }
+    for (; s < a.dim; ++s)
+    {
+        const double y = a.Y[s][i];
+        abar += y;
+        zbar += y * a.Z[s];
+    }
+    HelmOut o;
+    helmholtz_eval<true>(g, cells, cells3, abar, zbar, a.temp[i], a.rho[i], o);
+    if (a.u) a.u[i] = o.u;
+    if (a.cv) a.cv[i] = o.cv;
+    if (a.p) a.p[i] = o.p;
+    if (a.c) a.c[i] = o.c;
+    if (a.dpdT) a.dpdT[i] = o.dpdT;
 }
 
 __global__ void nnet_queue_reset(unsigned int* q, int n)
@@ -1310,48 +1374,62 @@ __device__ __forceinline__ int cost_bucket(uint32_t c)
     return b < NBUCKET ? b : NBUCKET - 1;
 }
 
-// bins[0 .. NBUCKET): counts; zeroed by nnet_queue_reset before the launch
-__global__ void __launch_bounds__(256) nnet_order_count(const uint32_t* __restrict__ cost, size_t n, unsigned* bins)
+// Dispatch key: the cost bucket, plus NBUCKET for a particle routed to the FP64 pass of a mixed
+// batch (nonzero fp64 hint).  Keys are laid out in descending order, so the routed particles form
+// the prefix [0, nH) of the order and the others [nH, n), each longest first.
+constexpr int NKEY = 2 * NBUCKET, BIN_STRIDE = 2 * NKEY + 4; // counts, offsets, range {0, nH, n}
+__device__ __forceinline__ int order_key(const uint32_t* cost, const uint32_t* hint, size_t i)
 {
-    __shared__ unsigned h[NBUCKET];
-    if (threadIdx.x < NBUCKET) h[threadIdx.x] = 0u;
-    __syncthreads();
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        atomicAdd(&h[cost_bucket(cost[i])], 1u);
-    __syncthreads();
-    if (threadIdx.x < NBUCKET && h[threadIdx.x]) atomicAdd(&bins[threadIdx.x], h[threadIdx.x]);
+    return (hint && hint[i] != 0u ? NBUCKET : 0) + (cost ? cost_bucket(cost[i]) : 0);
 }
 
-// bins[NBUCKET + b] = first position of bucket b (largest bucket first)
+// bins[0 .. NKEY): counts; zeroed by nnet_queue_reset before the launch
+__global__ void __launch_bounds__(256) nnet_order_count(const uint32_t* __restrict__ cost,
+                                                        const uint32_t* __restrict__ hint, size_t n, unsigned* bins)
+{
+    __shared__ unsigned h[NKEY];
+    if (threadIdx.x < NKEY) h[threadIdx.x] = 0u;
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        atomicAdd(&h[order_key(cost, hint, i)], 1u);
+    __syncthreads();
+    if (threadIdx.x < NKEY && h[threadIdx.x]) atomicAdd(&bins[threadIdx.x], h[threadIdx.x]);
+}
+
+// bins[NKEY + k] = first position of key k (largest key first); bins[2 NKEY ..] = {0, nH, n}
 __global__ void nnet_order_offsets(unsigned* bins)
 {
     if (threadIdx.x == 0)
     {
         unsigned acc = 0;
-        for (int b = NBUCKET - 1; b >= 0; --b)
+        for (int k = NKEY - 1; k >= 0; --k)
         {
-            bins[NBUCKET + b] = acc;
-            acc += bins[b];
+            if (k == NBUCKET - 1) bins[2 * NKEY + 1] = acc;
+            bins[NKEY + k] = acc;
+            acc += bins[k];
         }
+        bins[2 * NKEY]     = 0u;
+        bins[2 * NKEY + 2] = acc;
     }
 }
 
-// each 256-particle tile reserves its bucket ranges with one atomic per bucket
-__global__ void __launch_bounds__(256) nnet_order_scatter(const uint32_t* __restrict__ cost, size_t n, unsigned* bins,
-                                                         uint32_t* __restrict__ order)
+// each 256-particle tile reserves its key ranges with one atomic per key
+__global__ void __launch_bounds__(256) nnet_order_scatter(const uint32_t* __restrict__ cost,
+                                                          const uint32_t* __restrict__ hint, size_t n, unsigned* bins,
+                                                          uint32_t* __restrict__ order)
 {
-    __shared__ unsigned h[NBUCKET], base[NBUCKET];
+    __shared__ unsigned h[NKEY], base[NKEY];
     for (size_t t0 = (size_t)blockIdx.x * blockDim.x; t0 < n; t0 += (size_t)gridDim.x * blockDim.x)
     {
-        if (threadIdx.x < NBUCKET) h[threadIdx.x] = 0u;
+        if (threadIdx.x < NKEY) h[threadIdx.x] = 0u;
         __syncthreads();
-        const size_t i    = t0 + threadIdx.x;
-        const int    b    = i < n ? cost_bucket(cost[i]) : 0;
-        const unsigned r  = i < n ? atomicAdd(&h[b], 1u) : 0u;
+        const size_t   i = t0 + threadIdx.x;
+        const int      k = i < n ? order_key(cost, hint, i) : 0;
+        const unsigned r = i < n ? atomicAdd(&h[k], 1u) : 0u;
         __syncthreads();
-        if (threadIdx.x < NBUCKET && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&bins[NBUCKET + threadIdx.x], h[threadIdx.x]);
+        if (threadIdx.x < NKEY && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&bins[NKEY + threadIdx.x], h[threadIdx.x]);
         __syncthreads();
-        if (i < n) order[base[b] + r] = (uint32_t)i;
+        if (i < n) order[base[k] + r] = (uint32_t)i;
         __syncthreads();
     }
 }
@@ -1370,13 +1448,19 @@ struct nnet_ctx
     long        max_iter = 10000;
     // Helmholtz
     HelmGrid* d_grid  = nullptr;
-    double*   d_cells = nullptr;
-    bool      helm_ok = false;
-    // work-queue counters, one per batch of a call
+    double*   d_cells  = nullptr;
+    double*   d_cells3 = nullptr;
+    bool      helm_ok  = false;
+    // NCTR counters per batch of a call: work queues of the first pass, of the routed FP64 pass
+    // and of the redo pass; then the redo range {0, count}
+    static constexpr int NCTR = 5;
     unsigned int* d_queue = nullptr;
-    int           n_queue = 0;
+    int           n_queue = 0; // batches the counters cover
+    // mixed precision: the particles handed from the mixed pass to the FP64 redo pass
+    uint32_t* d_redo = nullptr;
+    size_t    n_redo = 0;
     int           n_cu    = 0;
-    // cost-ordered dispatch: per batch NBUCKET counts + NBUCKET offsets, and the particle order
+    // ordered dispatch: per batch BIN_STRIDE key counts, offsets and ranges, and the particle order
     unsigned int* d_bins   = nullptr;
     uint32_t*     d_order  = nullptr;
     size_t        n_order  = 0;
@@ -1387,6 +1471,11 @@ struct nnet_ctx
     hipStream_t          side[NSIDE]    = {};
     hipEvent_t           side_ev[NSIDE] = {};
     hipEvent_t           fork_ev        = nullptr;
+    // a mixed batch with an fp64 hint runs its routed FP64 pass on aux[k] concurrently with the
+    // mixed pass on the batch's stream (side[k] or the caller's)
+    hipStream_t aux[NSIDE]      = {};
+    hipEvent_t  aux_fork[NSIDE] = {};
+    hipEvent_t  aux_join[NSIDE] = {};
     std::vector<void*> owned;
 };
 
@@ -1756,7 +1845,10 @@ extern "C"
         }
         for (int k = 0; k < nnet_ctx::NSIDE; ++k)
             if (hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) != hipSuccess ||
-                hipEventCreateWithFlags(&c->side_ev[k], hipEventDisableTiming) != hipSuccess)
+                hipEventCreateWithFlags(&c->side_ev[k], hipEventDisableTiming) != hipSuccess ||
+                hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&c->aux_fork[k], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&c->aux_join[k], hipEventDisableTiming) != hipSuccess)
             {
                 nnet_ctx_destroy(c);
                 return NNET_E_HIP;
@@ -1777,13 +1869,18 @@ extern "C"
         (void)hipSetDevice(c->device);
         if (c->d_grid) (void)hipFree(c->d_grid);
         if (c->d_cells) (void)hipFree(c->d_cells);
+        if (c->d_cells3) (void)hipFree(c->d_cells3);
         if (c->d_queue) (void)hipFree(c->d_queue);
         if (c->d_bins) (void)hipFree(c->d_bins);
         if (c->d_order) (void)hipFree(c->d_order);
+        if (c->d_redo) (void)hipFree(c->d_redo);
         for (int k = 0; k < nnet_ctx::NSIDE; ++k)
         {
             if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
             if (c->side_ev[k]) (void)hipEventDestroy(c->side_ev[k]);
+            if (c->aux[k]) (void)hipStreamDestroy(c->aux[k]);
+            if (c->aux_fork[k]) (void)hipEventDestroy(c->aux_fork[k]);
+            if (c->aux_join[k]) (void)hipEventDestroy(c->aux_join[k]);
         }
         if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
         for (void* p : c->owned)
@@ -1842,11 +1939,29 @@ extern "C"
                     o[4 * qq + 3]   = a[(size_t)(i + 1) * HELM_JMAX + (j + 1)];
                 }
             }
+        // bicubic dP/drho stencil per cell, coefficient order of helmholtz.cpp:587-602
+        const int           ks3[4] = {9, 11, 10, 12}; // dpdf, dpdft, dpdfd, dpdfdt
+        std::vector<double> st3(cells * HELM_CELL3, 0.);
+        for (int i = 0; i + 1 < HELM_IMAX; ++i)
+            for (int j = 0; j + 1 < HELM_JMAX; ++j)
+            {
+                double* o = &st3[((size_t)i * HELM_JMAX + j) * HELM_CELL3];
+                for (int qq = 0; qq < 4; ++qq)
+                {
+                    const double* a = &arr[(size_t)ks3[qq] * cells];
+                    o[4 * qq + 0]   = a[(size_t)(i + 0) * HELM_JMAX + (j + 0)];
+                    o[4 * qq + 1]   = a[(size_t)(i + 1) * HELM_JMAX + (j + 0)];
+                    o[4 * qq + 2]   = a[(size_t)(i + 0) * HELM_JMAX + (j + 1)];
+                    o[4 * qq + 3]   = a[(size_t)(i + 1) * HELM_JMAX + (j + 1)];
+                }
+            }
         HIP_TRY(c, hipSetDevice(c->device));
         if (!c->d_grid) HIP_TRY(c, hipMalloc(&c->d_grid, sizeof(HelmGrid)));
         if (!c->d_cells) HIP_TRY(c, hipMalloc(&c->d_cells, st.size() * sizeof(double)));
+        if (!c->d_cells3) HIP_TRY(c, hipMalloc(&c->d_cells3, st3.size() * sizeof(double)));
         HIP_TRY(c, hipMemcpy(c->d_grid, &g, sizeof(HelmGrid), hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy(c->d_cells, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->d_cells3, st3.data(), st3.size() * sizeof(double), hipMemcpyHostToDevice));
         c->helm_ok = true;
         return NNET_OK;
     }
@@ -1921,14 +2036,16 @@ extern "C"
             HIP_TRY(c, hipStreamSynchronize(st));
             if (c->d_queue) HIP_TRY(c, hipFree(c->d_queue));
             c->d_queue = nullptr;
-            HIP_TRY(c, hipMalloc(&c->d_queue, sizeof(unsigned int) * std::max(nb, 8)));
+            HIP_TRY(c, hipMalloc(&c->d_queue, sizeof(unsigned int) * nnet_ctx::NCTR * std::max(nb, 8)));
             c->n_queue = std::max(nb, 8);
         }
         // cost-ordered dispatch buffers: grow only when a call needs more than before
         {
             size_t need = 0;
             for (int bi = 0; bi < nb; ++bi)
-                if (batches[bi].cost_hint) need += batches[bi].n;
+                if (batches[bi].cost_hint ||
+                    (batches[bi].fp64_hint && batches[bi].precision == NNET_PRECISION_MIXED))
+                    need += batches[bi].n;
             if (need > c->n_order || (need > 0 && nb > c->n_bins))
             {
                 HIP_TRY(c, hipStreamSynchronize(st));
@@ -1943,9 +2060,23 @@ extern "C"
                 {
                     if (c->d_bins) HIP_TRY(c, hipFree(c->d_bins));
                     c->d_bins = nullptr;
-                    HIP_TRY(c, hipMalloc(&c->d_bins, sizeof(unsigned int) * 2 * NBUCKET * std::max(nb, 8)));
+                    HIP_TRY(c, hipMalloc(&c->d_bins, sizeof(unsigned int) * BIN_STRIDE * std::max(nb, 8)));
                     c->n_bins = std::max(nb, 8);
                 }
+            }
+        }
+        // redo lists of the mixed batches: grow only when a call needs more than before
+        {
+            size_t need = 0;
+            for (int bi = 0; bi < nb; ++bi)
+                if (batches[bi].precision == NNET_PRECISION_MIXED) need += batches[bi].n;
+            if (need > c->n_redo)
+            {
+                HIP_TRY(c, hipStreamSynchronize(st));
+                if (c->d_redo) HIP_TRY(c, hipFree(c->d_redo));
+                c->d_redo = nullptr;
+                HIP_TRY(c, hipMalloc(&c->d_redo, sizeof(uint32_t) * need));
+                c->n_redo = need;
             }
         }
         // validate every batch before anything is enqueued
@@ -1977,7 +2108,7 @@ extern "C"
             for (int k = 0; k < std::min(n_live, (int)nnet_ctx::NSIDE); ++k)
                 HIP_TRY(c, hipStreamWaitEvent(c->side[k], c->fork_ev, 0));
         }
-        size_t order_off = 0;
+        size_t order_off = 0, redo_off = 0;
         int    live      = 0;
         for (int bi = 0; bi < nb; ++bi)
         {
@@ -2001,54 +2132,103 @@ extern "C"
             bd.status = b.status, bd.substeps = b.substeps, bd.newton = b.newton, bd.fp64_solves = b.fp64_solves;
             for (int z = 0; z < d; ++z)
                 bd.Y[z] = b.Y[z];
-            const NetDev& nd = b.network->dev;
-            bd.queue         = c->d_queue + bi;
-            hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, bst, bd.queue, 1);
-            bd.order = nullptr;
-            if (b.cost_hint)
+            const NetDev& nd  = b.network->dev;
+            unsigned int* ctr = c->d_queue + (size_t)nnet_ctx::NCTR * bi;
+            bd.queue          = ctr;
+            hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, bst, ctr, nnet_ctx::NCTR);
+            const bool mix = b.precision == NNET_PRECISION_MIXED;
+            if (mix)
             {
-                unsigned int* bins  = c->d_bins + (size_t)2 * NBUCKET * bi;
+                bd.redo       = c->d_redo + redo_off;
+                bd.redo_count = ctr + 4;
+                redo_off += b.n;
+            }
+            // ordered dispatch: longest first by the cost hint; in a mixed batch, the particles with
+            // a nonzero fp64 hint form the prefix [0, nH) of the order, the rest [nH, n)
+            const uint32_t*     route = mix ? b.fp64_hint : nullptr;
+            const unsigned int* range = nullptr;
+            bd.order                  = nullptr;
+            if (b.cost_hint || route)
+            {
+                unsigned int* bins  = c->d_bins + (size_t)BIN_STRIDE * bi;
                 uint32_t*     order = c->d_order + order_off;
                 order_off += b.n;
                 const unsigned blocks = (unsigned)std::min<size_t>((b.n + 255) / 256, 2048);
-                hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, bst, bins, 2 * NBUCKET);
-                hipLaunchKernelGGL(nnet_order_count, dim3(blocks), dim3(256), 0, bst, b.cost_hint, b.n, bins);
+                hipLaunchKernelGGL(nnet_queue_reset, dim3((BIN_STRIDE + 255) / 256), dim3(256), 0, bst, bins,
+                                   BIN_STRIDE);
+                hipLaunchKernelGGL(nnet_order_count, dim3(blocks), dim3(256), 0, bst, b.cost_hint, route, b.n, bins);
                 hipLaunchKernelGGL(nnet_order_offsets, dim3(1), dim3(64), 0, bst, bins);
-                hipLaunchKernelGGL(nnet_order_scatter, dim3(blocks), dim3(256), 0, bst, b.cost_hint, b.n, bins, order);
+                hipLaunchKernelGGL(nnet_order_scatter, dim3(blocks), dim3(256), 0, bst, b.cost_hint, route, b.n, bins,
+                                   order);
                 bd.order = order;
+                range    = bins + 2 * NKEY; // {0, nH, n}
             }
-            const int kind = b.network->kind;
-            auto      launch = [&](auto kern, int nt) -> int
+            const int         kind = b.network->kind;
+            const int         ak   = fork ? (live - 1) % nnet_ctx::NSIDE : 0;
+            const hipStream_t ast  = c->aux[ak];
+            hipError_t        herr = hipSuccess;
+            auto              chk  = [&](hipError_t e)
+            {
+                if (e != hipSuccess && herr == hipSuccess) herr = e;
+            };
+            auto              launch = [&](auto kern, int nt, const BatchDev& bdv, const StepParams& spv, hipStream_t s)
             {
                 int per_cu = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, 0) != hipSuccess || per_cu < 1)
                     per_cu = 1;
                 const size_t wgs = std::min<size_t>(b.n, (size_t)per_cu * c->n_cu);
-                hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(nt), 0, bst, nd, ed, bd, sp);
-                return 0;
+                hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(nt), 0, s, nd, ed, bdv, spv);
             };
-            const bool mix      = b.precision == NNET_PRECISION_MIXED;
-            auto       dispatch = [&](auto netc)
+            // mixed batch: the routed particles [0, nH) go to an FP64 pass on the aux stream while
+            // the mixed pass takes [nH, n); the particles the mixed pass gives up go to a last FP64
+            // pass over its redo list (rho_m1 already updated by the mixed pass)
+            BatchDev routed_bd  = bd;
+            routed_bd.queue     = ctr + 1;
+            routed_bd.qrange    = range;
+            routed_bd.fp64_pass = 1;
+            BatchDev mixed_bd   = bd;
+            mixed_bd.qrange     = route ? range + 1 : nullptr;
+            BatchDev redo_bd    = bd;
+            redo_bd.queue       = ctr + 2;
+            redo_bd.order       = bd.redo;
+            redo_bd.qrange      = ctr + 3; // {0, redo count}
+            redo_bd.fp64_pass   = 1;
+            StepParams redo_sp  = sp;
+            redo_sp.use_drhodt  = 0;
+            auto dispatch       = [&](auto netc, auto eosc)
             {
-                constexpr int NV = decltype(netc)::value;
-                if (b.eos == NNET_EOS_IDEAL)
+                constexpr int NV = decltype(netc)::value, EV = decltype(eosc)::value;
+                if (!mix)
                 {
-                    if (mix) launch(nnet_solve_kernel<NV, NNET_EOS_IDEAL, true>, Cfg<NV>::NT);
-                    else
-                        launch(nnet_solve_kernel<NV, NNET_EOS_IDEAL, false>, Cfg<NV>::NT);
+                    launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, bd, sp, bst);
+                    return;
                 }
-                else
+                if (route)
                 {
-                    if (mix) launch(nnet_solve_kernel<NV, NNET_EOS_HELMHOLTZ, true>, Cfg<NV>::NT);
-                    else
-                        launch(nnet_solve_kernel<NV, NNET_EOS_HELMHOLTZ, false>, Cfg<NV>::NT);
+                    chk(hipEventRecord(c->aux_fork[ak], bst));
+                    chk(hipStreamWaitEvent(ast, c->aux_fork[ak], 0));
+                    launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, routed_bd, sp, ast);
+                }
+                launch(nnet_solve_kernel<NV, EV, true>, Cfg<NV>::NT, mixed_bd, sp, bst);
+                launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, redo_bd, redo_sp, bst);
+                if (route)
+                {
+                    chk(hipEventRecord(c->aux_join[ak], ast));
+                    chk(hipStreamWaitEvent(bst, c->aux_join[ak], 0));
                 }
             };
-            if (kind == 14) dispatch(std::integral_constant<int, 14>{});
+            auto dispatch_eos = [&](auto netc)
+            {
+                if (b.eos == NNET_EOS_IDEAL) dispatch(netc, std::integral_constant<int, NNET_EOS_IDEAL>{});
+                else
+                    dispatch(netc, std::integral_constant<int, NNET_EOS_HELMHOLTZ>{});
+            };
+            if (kind == 14) dispatch_eos(std::integral_constant<int, 14>{});
             else if (kind == 86)
-                dispatch(std::integral_constant<int, 86>{});
+                dispatch_eos(std::integral_constant<int, 86>{});
             else
-                dispatch(std::integral_constant<int, 87>{});
+                dispatch_eos(std::integral_constant<int, 87>{});
+            HIP_TRY(c, herr);
             HIP_TRY(c, hipGetLastError());
         }
         if (fork)
@@ -2064,6 +2244,44 @@ extern "C"
                           int use_drhodt)
     {
         int rc = nnet_compute(c, batches, nb, hydro_dt, previous_dt, use_drhodt, nullptr);
+        if (rc != NNET_OK) return rc;
+        HIP_TRY(c, hipDeviceSynchronize());
+        return NNET_OK;
+    }
+
+    int nnet_helm_eos(nnet_ctx* c, size_t n, int dimension, const double* Z, const double* temp, const double* rho,
+                      const double* const* Y, double* u, double* cv, double* p, double* cs, double* dpdT, void* stream)
+    {
+        if (!c) return NNET_E_ARG;
+        if (dimension < 1 || dimension > HELM_EOS_MAX_DIM)
+            return fail(c, NNET_E_ARG, "nnet_helm_eos: dimension must be in [1, " + std::to_string(HELM_EOS_MAX_DIM) + "]");
+        if (!Z || !Y) return fail(c, NNET_E_ARG, "nnet_helm_eos: Z and Y are required");
+        if (!c->helm_ok) return fail(c, NNET_E_STATE, "no Helmholtz table loaded");
+        if (n == 0) return NNET_OK;
+        if (!temp || !rho) return fail(c, NNET_E_ARG, "nnet_helm_eos: temp and rho are required");
+        if (n > (size_t)0x7fffffff * 256) return fail(c, NNET_E_ARG, "nnet_helm_eos: too many particles for one call");
+        HelmEosArgs a{};
+        a.n = n, a.dim = dimension, a.temp = temp, a.rho = rho;
+        a.u = u, a.cv = cv, a.p = p, a.c = cs, a.dpdT = dpdT;
+        for (int j = 0; j < dimension; ++j)
+        {
+            if (!Y[j]) return fail(c, NNET_E_ARG, "nnet_helm_eos: null abundance array");
+            a.Y[j] = Y[j];
+            a.Z[j] = Z[j];
+        }
+        HIP_TRY(c, hipSetDevice(c->device));
+        hipLaunchKernelGGL(nnet_helm_eos_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           (hipStream_t)stream, a, (const HelmGrid*)c->d_grid, (const double*)c->d_cells,
+                           (const double*)c->d_cells3);
+        HIP_TRY(c, hipGetLastError());
+        return NNET_OK;
+    }
+
+    int nnet_helm_eos_sync(nnet_ctx* c, size_t n, int dimension, const double* Z, const double* temp,
+                           const double* rho, const double* const* Y, double* u, double* cv, double* p, double* cs,
+                           double* dpdT)
+    {
+        int rc = nnet_helm_eos(c, n, dimension, Z, temp, rho, Y, u, cv, p, cs, dpdT, nullptr);
         if (rc != NNET_OK) return rc;
         HIP_TRY(c, hipDeviceSynchronize());
         return NNET_OK;

@@ -285,6 +285,7 @@ struct Options
     uint32_t* newton    = nullptr; // device (optional)
     uint32_t* fp64_solves = nullptr; // device (optional, mixed precision)
     const uint32_t* cost_hint = nullptr; // device (optional): expected cost per particle, e.g. newton of the last call
+    const uint32_t* fp64_hint = nullptr; // device (optional, mixed precision): nonzero -> FP64 path, e.g. fp64_solves of the last call
 };
 
 template<bool use_gpu, class Data, typename Float, class List, int KIND, class Eos, class nseFunction = void*>
@@ -323,6 +324,7 @@ void computeNuclearReactions(Data& n, size_t firstIndex, size_t lastIndex, const
     b.newton    = opt.newton;
     b.fp64_solves = opt.fp64_solves;
     b.cost_hint   = opt.cost_hint;
+    b.fp64_hint   = opt.fp64_hint;
     if constexpr (std::is_same_v<Eos, eos::IdealGasFunctor>)
     {
         b.eos = NNET_EOS_IDEAL;
@@ -337,6 +339,34 @@ void computeNuclearReactions(Data& n, size_t firstIndex, size_t lastIndex, const
     }
     if (use_drhodt && b.rho_m1 == nullptr) use_drhodt = false; // the reference disables it with a warning
     detail::check(c, nnet_compute_sync(c, &b, 1, hydro_dt, previous_dt, use_drhodt ? 1 : 0), "nnet_compute");
+}
+
+/* Post-step EOS refresh, mirroring
+ *   template<bool use_gpu, class Data, class Vector>
+ *   void nnet::parallel::computeHelmEOS(Data& n, size_t firstIndex, size_t lastIndex, const Vector& Z)
+ *                                       (include/nnet/parallel/parallel_nuclear_net.hpp:191-261):
+ * fields Y, temp, rho (read) and u, cv, p, c, dpdT (written) of n.devData over
+ * [firstIndex, lastIndex), with Z.size() >= n.numSpecies charges.  Synchronises before returning. */
+template<bool use_gpu, class Data, class Vector>
+void computeHelmEOS(Data& n, size_t firstIndex, size_t lastIndex, const Vector& Z, const Options& opt = Options{})
+{
+    static_assert(use_gpu, "this implementation runs on the GPU; the reference's CPU branch is not provided");
+    if (lastIndex <= firstIndex) return;
+    auto&                       reg = detail::Registry::get();
+    std::lock_guard<std::mutex> lk(reg.mu);
+    nnet_ctx*                   c   = reg.context(opt.device);
+
+    auto&                     dev       = n.devData;
+    const int                 dimension = (int)n.numSpecies;
+    std::vector<const double*> Y(dimension);
+    for (int i = 0; i < dimension; ++i)
+        Y[i] = detail::raw(dev.Y[i].data()) + firstIndex;
+    std::vector<double> Zd(Z.begin(), Z.begin() + dimension);
+    auto at = [&](auto& field) { return detail::raw(field.data()) + firstIndex; };
+    detail::check(c,
+                  nnet_helm_eos_sync(c, lastIndex - firstIndex, dimension, Zd.data(), at(dev.temp), at(dev.rho),
+                                     Y.data(), at(dev.u), at(dev.cv), at(dev.p), at(dev.c), at(dev.dpdT)),
+                  "nnet_helm_eos");
 }
 } // namespace parallel
 } // namespace nnet_amd

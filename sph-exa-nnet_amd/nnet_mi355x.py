@@ -42,6 +42,7 @@ class Batch(C.Structure):
         ("newton", C.c_void_p),
         ("fp64_solves", C.c_void_p),
         ("cost_hint", C.c_void_p),
+        ("fp64_hint", C.c_void_p),
     ]
 
 
@@ -67,6 +68,9 @@ def lib():
         L.nnet_network_reactions.argtypes = [vp, ip, ip, ip, ip, ip, ip]
         L.nnet_compute.argtypes = [vp, C.POINTER(Batch), C.c_int, C.c_double, C.c_double, C.c_int, vp]
         L.nnet_compute_sync.argtypes = [vp, C.POINTER(Batch), C.c_int, C.c_double, C.c_double, C.c_int]
+        eos_args = [vp, C.c_size_t, C.c_int, dp, vp, vp, C.POINTER(vp), vp, vp, vp, vp, vp]
+        L.nnet_helm_eos.argtypes = eos_args + [vp]
+        L.nnet_helm_eos_sync.argtypes = eos_args
         _lib = L
     return _lib
 
@@ -74,7 +78,7 @@ def lib():
 EXPORTS = [
     "nnet_ctx_create", "nnet_ctx_destroy", "nnet_last_error", "nnet_ctx_load_helmholtz",
     "nnet_network_create", "nnet_network_builtin", "nnet_network_destroy", "nnet_network_reactions",
-    "nnet_compute", "nnet_compute_sync", "nnet_ctx_set_max_iterations",
+    "nnet_compute", "nnet_compute_sync", "nnet_ctx_set_max_iterations", "nnet_helm_eos", "nnet_helm_eos_sync",
 ]
 
 
@@ -122,7 +126,7 @@ class Context:
 
     def make_batch(self, net, state, eos=EOS_IDEAL, mu=10.0, Z=None, precision=PRECISION_FP64):
         """state: dict of torch device tensors rho, rho_m1, Y (d x n), temp, dt, status, substeps, newton
-        and (optional) fp64_solves and cost_hint."""
+        and (optional) fp64_solves, cost_hint and fp64_hint."""
         b = Batch()
         b.network = self.network(net).value
         b.eos = eos
@@ -144,6 +148,7 @@ class Context:
         b.status, b.substeps, b.newton = ptr("status"), ptr("substeps"), ptr("newton")
         b.fp64_solves = ptr("fp64_solves")
         b.cost_hint = ptr("cost_hint")
+        b.fp64_hint = ptr("fp64_hint")
         yp = (C.c_void_p * Y.shape[0])(*[Y[s].data_ptr() for s in range(Y.shape[0])])
         b.Y = C.cast(yp, C.POINTER(C.c_void_p))
         b._ykeep = yp
@@ -157,6 +162,22 @@ class Context:
         else:
             rc = lib().nnet_compute(self.h, arr, len(batches), hydro_dt, prev, int(use_drhodt), stream)
         self.check(rc, "nnet_compute")
+
+    def helm_eos(self, Z, temp, rho, Y, out, stream=None, sync=True):
+        """Post-step EOS refresh (computeHelmEOS): Y is a (dim x n) device tensor (or a list of dim
+        per-species device tensors), temp and rho device tensors of n doubles, `out` a dict whose
+        keys among u, cv, p, c, dpdT name the device tensors to fill (others are skipped)."""
+        rows = [Y[s] for s in range(len(Y))]
+        n = temp.shape[0]
+        zarr = (C.c_double * len(Z))(*[float(z) for z in Z])
+        yp = (C.c_void_p * len(rows))(*[r.data_ptr() for r in rows])
+        o = [out[k].data_ptr() if out.get(k) is not None else None for k in ("u", "cv", "p", "c", "dpdT")]
+        args = [self.h, n, len(rows), zarr, temp.data_ptr(), rho.data_ptr(), yp] + o
+        if sync and stream is None:
+            rc = lib().nnet_helm_eos_sync(*args)
+        else:
+            rc = lib().nnet_helm_eos(*args, stream)
+        self.check(rc, "nnet_helm_eos")
 
     def close(self):
         if self.h is not None and self.h.value:

@@ -1,12 +1,14 @@
-// Drives nnet_amd::parallel::computeNuclearReactions (the C++ mirror of the reference's
-// nnet::parallel::computeNuclearReactions) the way SPH-EXA calls it: a Data object whose devData
-// holds device arrays rho, rho_m1, temp, dt and Y[numSpecies].
+// Drives nnet_amd::parallel::computeNuclearReactions and, for the Helmholtz EOS, the post-step
+// nnet_amd::parallel::computeHelmEOS (the C++ mirrors of the reference's nnet::parallel
+// functions of the same names) the way SPH-EXA calls them: a Data object whose devData holds
+// device arrays rho, rho_m1, temp, dt, u, cv, p, c, dpdT and Y[numSpecies].
 //
 //   shim_driver <net 14|86|87> <eos 0|1> <in.bin> <out.bin> [helm_table.bin]
 //
 // in.bin : int64 n, double hydro_dt, double mu, then rho[n], temp[n], dt[n], Y[d][n],
 //          and for the Helmholtz EOS Z[d]
-// out.bin: temp[n], dt[n], Y[d][n], status[n] (uint8), substeps[n] (uint32), newton[n] (uint32)
+// out.bin: temp[n], dt[n], Y[d][n], status[n] (uint8), substeps[n] (uint32), newton[n] (uint32),
+//          and for the Helmholtz EOS u[n], cv[n], p[n], c[n], dpdT[n] after the solve
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -52,7 +54,7 @@ struct DeviceArray
 
 struct DeviceData
 {
-    DeviceArray<double>              rho, rho_m1, temp, dt;
+    DeviceArray<double>              rho, rho_m1, temp, dt, u, cv, p, c, dpdT;
     std::vector<DeviceArray<double>> Y;
 };
 struct ParticleData
@@ -104,6 +106,8 @@ int main(int argc, char** argv)
         dv.Y.emplace_back(n);
         dv.Y.back().upload(Y.data() + (size_t)s * n);
     }
+    for (auto* a : {&dv.u, &dv.cv, &dv.p, &dv.c, &dv.dpdT})
+        *a = DeviceArray<double>(n);
     DeviceArray<uint8_t>          st(n);
     DeviceArray<uint32_t>         sub(n), newton(n);
     nnet_amd::parallel::Options   opt;
@@ -118,6 +122,7 @@ int main(int argc, char** argv)
             if (net == 14) run(pd, n, hdt, nnet_amd::net14::ComputeReactionRatesFunctor{}, helm, 14, opt);
             else if (net == 86) run(pd, n, hdt, nnet_amd::net86::ComputeReactionRatesFunctor{}, helm, 86, opt);
             else run(pd, n, hdt, nnet_amd::net87::ComputeReactionRatesFunctor{}, helm, 87, opt);
+            nnet_amd::parallel::computeHelmEOS<true>(pd, 0, n, Z);
         }
         else
         {
@@ -149,6 +154,13 @@ int main(int argc, char** argv)
     std::fwrite(hs.data(), 1, n, o);
     std::fwrite(hsub.data(), 4, n, o);
     std::fwrite(hnew.data(), 4, n, o);
+    if (eos == 1)
+        for (auto* a : {&dv.u, &dv.cv, &dv.p, &dv.c, &dv.dpdT})
+        {
+            std::vector<double> h(n);
+            a->download(h.data());
+            std::fwrite(h.data(), 8, n, o);
+        }
     std::fclose(o);
     std::printf("shim_driver: net%d eos %d n=%lld done\n", net, eos, (long long)n);
     return 0;

@@ -156,8 +156,30 @@ def make_golden_helm(ref):
         np.savez_compressed(os.path.join(HERE, f"helm_particles_c{cid}.npz"), Z=cfg["Z"], net=net,
                             hydro_dt=cfg["hydro_dt"], digest=table_digest(table),
                             **{k: np.array(v) for k, v in res.items()})
+    make_golden_helm_eos_batch(ref)
     print("Helmholtz golden vectors written")
 
 
+def make_golden_helm_eos_batch(ref):
+    """The reference's post-step EOS refresh (computeHelmholtzImpl) on 40 particles of each of the
+    net14 / net86 / net87 Helmholtz distributions plus the table's corners (temperature and
+    Ye*rho below and above the grid, where the cell index is clamped)."""
+    table = os.path.join(ROOT, "sph-exa-nnet_amd", "data", "helm_table.bin")
+    res = {}
+    for cid in (2, 3, 4):
+        cfg = workloads.config(cid, n=40, seed=57)
+        T, rho, Y = cfg["temp"].copy(), cfg["rho"].copy(), cfg["Y"].copy()
+        T[:4] = [5e2, 5e2, 3e13, 3e13]
+        rho[:4] = [1e-13, 1e16, 1e-13, 1e16]
+        out = ref.helm_eos_batch(cfg["Z"], T, rho, Y)
+        net = cfg["net"]
+        res.update({f"T{net}": T, f"rho{net}": rho, f"Y{net}": Y, f"Z{net}": cfg["Z"],
+                    f"out{net}": np.array(out)})
+    np.savez_compressed(os.path.join(HERE, "helm_eos_batch.npz"), digest=table_digest(table), **res)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--helm-eos-batch"]:
+        make_golden_helm_eos_batch(Reference())
+    else:
+        main()

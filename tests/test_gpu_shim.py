@@ -31,6 +31,8 @@ def _write_input(path, cfg):
 
 
 def _read_output(path, n, d):
+    """(Y, T, dt, status, substeps, newton) and, for the Helmholtz EOS, the refreshed
+    (u, cv, p, c, dpdT) of computeHelmEOS (else None)."""
     raw = open(path, "rb").read()
     o = 0
     T = np.frombuffer(raw, np.float64, n, o); o += 8 * n
@@ -38,8 +40,9 @@ def _read_output(path, n, d):
     Y = np.frombuffer(raw, np.float64, d * n, o).reshape(d, n); o += 8 * d * n
     st = np.frombuffer(raw, np.uint8, n, o); o += n
     sub = np.frombuffer(raw, np.uint32, n, o); o += 4 * n
-    new = np.frombuffer(raw, np.uint32, n, o)
-    return Y, T, dt, st, sub, new
+    new = np.frombuffer(raw, np.uint32, n, o); o += 4 * n
+    eos = np.frombuffer(raw, np.float64, 5 * n, o).reshape(5, n) if len(raw) >= o + 40 * n else None
+    return (Y, T, dt, st, sub, new), eos
 
 
 @pytest.mark.parametrize("cid,n", [(2, 512), (3, 192)])
@@ -56,7 +59,7 @@ def test_shim_matches_cabi_and_oracle(tmp_path, oracle_lib, cid, n):
     _write_input(fin, cfg)
     print(f"shim_driver net{cfg['net']} n={n}", flush=True)
     subprocess.run([DRIVER, str(cfg["net"]), str(cfg["eos"]), fin, fout, TABLE], check=True, timeout=120)
-    s = _read_output(fout, n, d)
+    s, s_eos = _read_output(fout, n, d)
 
     ctx = nm.Context(0, helm_table=TABLE)
     st = nm.device_state(torch, cfg["net"], cfg["rho"], cfg["temp"], cfg["Y"], cfg["dt"])
@@ -65,9 +68,14 @@ def test_shim_matches_cabi_and_oracle(tmp_path, oracle_lib, cid, n):
     torch.cuda.synchronize()
     g = (st["Y"].cpu().numpy(), st["temp"].cpu().numpy(), st["dt"].cpu().numpy(), st["status"].cpu().numpy(),
          st["substeps"].cpu().numpy().astype(np.uint32), st["newton"].cpu().numpy().astype(np.uint32))
+    # computeHelmEOS on the solved state through the C-ABI: identical to the mirror's
+    out = {k: torch.empty(n, dtype=torch.float64, device="cuda:0") for k in ("u", "cv", "p", "c", "dpdT")}
+    ctx.helm_eos(cfg["Z"], st["temp"], st["rho"], st["Y"], out)
+    g_eos = np.array([out[k].cpu().numpy() for k in ("u", "cv", "p", "c", "dpdT")])
     ctx.close()
     for a, c in zip(s, g):
         assert np.array_equal(np.asarray(a), np.asarray(c), equal_nan=True)
+    assert s_eos is not None and np.array_equal(s_eos, g_eos, equal_nan=True)
 
     oracle_lib.helm_load(TABLE)
     Y = np.ascontiguousarray(cfg["Y"].copy())

@@ -150,3 +150,39 @@ def test_helmholtz_golden_particles(oracle_lib, cid):
                                               eos_kind=1, Z=g["Z"], max_iter=10000)
         assert np.array_equal(Y.view(np.int64), g["Y"][k].view(np.int64)), (cid, k)
         assert T == g["T"][k] and dt == g["dt"][k] and np.array_equal(st[:4], g["stats"][k][:4]), (cid, k)
+
+
+@needs_table
+def test_helm_eos_batch_golden(oracle_lib):
+    """computeHelmholtzImpl restated (oracle_helm_eos_batch) against the reference's outputs,
+    bit for bit, including the out-of-table corners and a NaN sound speed."""
+    g = _fixture("helm_eos_batch.npz")
+    oracle_lib.helm_load(TABLE)
+    for net in (14, 86, 87):
+        got = np.array(oracle_lib.helm_eos_batch(g[f"Z{net}"], g[f"T{net}"], g[f"rho{net}"], g[f"Y{net}"]))
+        assert np.array_equal(got.view(np.int64), g[f"out{net}"].view(np.int64)), net
+
+
+@needs_table
+def test_helm_eos_batch_vs_reference(oracle_lib, ref_lib):
+    if not ref_lib.has_helmholtz:
+        pytest.skip("reference built without a Helmholtz table")
+    oracle_lib.helm_load(TABLE)
+    for cid in (2, 3, 4):
+        cfg = workloads.config(cid, n=2000, seed=99 + cid)
+        a = np.array(oracle_lib.helm_eos_batch(cfg["Z"], cfg["temp"], cfg["rho"], cfg["Y"]))
+        b = np.array(ref_lib.helm_eos_batch(cfg["Z"], cfg["temp"], cfg["rho"], cfg["Y"]))
+        assert np.array_equal(a.view(np.int64), b.view(np.int64)), cid
+
+
+@needs_table
+def test_helm_eos_batch_is_the_pointwise_eos(oracle_lib):
+    """The batch form equals the single-point EOS (oracle_eos, pinned separately) at abar_ =
+    sum(Y), zbar_ = sum(Y Z): the sums and the field mapping u, cv, p, c, dpdT are right."""
+    oracle_lib.helm_load(TABLE)
+    cfg = workloads.config(4, n=50, seed=5)
+    out = np.array(oracle_lib.helm_eos_batch(cfg["Z"], cfg["temp"], cfg["rho"], cfg["Y"]))
+    for i in range(50):
+        e = oracle_lib.eos(1, cfg["Y"][:, i], cfg["temp"][i], cfg["rho"][i], Z=cfg["Z"])
+        # oracle_eos order: cv, dpdT, p, cp, c, u, ...
+        assert np.array_equal(out[:, i], [e[5], e[0], e[2], e[4], e[1]], equal_nan=True)
