@@ -32,13 +32,18 @@ TABLE = os.path.join(ROOT, "sph-exa-nnet_amd", "data", "helm_table.bin")
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix)
 
 # HBM bytes per particle-step of the solve kernels on the net86 C3 distribution, from rocprofv3
-# --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of `bench.py --steps 1 --warmup 0 --no-cpu` (one
-# dispatch per precision each), scaled by the same counters on a plain 512 MiB device copy
-# (`bench.py --copy-calibration 512`: FETCH_SIZE reads 0.5x and WRITE_SIZE 1.0x of the copied
-# bytes).  FP64: 2 x 9,567,955 KiB fetched + 25,542,400 KiB written; mixed: 2 x 60,692,647 KiB
-# fetched + 82,258,290 KiB written, per 2,097,152 particles.
-TRAFFIC_BYTES_PER_PARTICLE_STEP = {(3, "fp64"): (2 * 9567955.06 + 25542399.59) * 1024 / 2097152,
-                                   (3, "mixed"): (2 * 60692647.44 + 82258289.88) * 1024 / 2097152}
+# --pmc FETCH_SIZE and --pmc WRITE_SIZE runs, scaled by the same counters on a plain 512 MiB device
+# copy (`bench.py --copy-calibration 512`: FETCH_SIZE reads 0.5x and WRITE_SIZE 1.0x of the copied
+# bytes).  FP64 (`--steps 1 --warmup 0 --no-cpu`, one dispatch): 2 x 4,839,345 KiB fetched +
+# 3,999,784 KiB written.  Mixed (`--precision mixed --steps 1 --warmup 1 --no-cpu`, the hinted step:
+# routed FP64 pass + mixed pass + redo pass): 2 x 32,627,335 KiB fetched + 8,997,707 KiB written;
+# per 2,097,152 particles.
+TRAFFIC_BYTES_PER_PARTICLE_STEP = {(3, "fp64"): (2 * 4839345.31 + 3999784.22) * 1024 / 2097152,
+                                   (3, "mixed"): (2 * 32627334.63 + 8997706.63) * 1024 / 2097152}
+
+# The same for the EOS refresh kernel over the C3 particle set (`--workload helm-eos --steps 2
+# --warmup 0 --no-cpu`): 2 x 755,378 KiB fetched + 81,920 KiB written per 2,097,152 particles.
+EOS_TRAFFIC_BYTES_PER_PARTICLE = {3: (2 * 755378.03 + 81920.0) * 1024 / 2097152}
 
 
 def flops_per_iter(net):
@@ -180,7 +185,9 @@ def run_helm_eos(args, torch, nm):
         "config": {"workload": f"EOS refresh over the {cfg['label']} particle set", "particles": n,
                    "kernel_ms": kernel_s * 1e3, "algorithmic_bytes_per_particle": bytes_per_particle},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8e12, "traffic": None},
+                     "frac": achieved / 8e12,
+                     "traffic": EOS_TRAFFIC_BYTES_PER_PARTICLE[cid] * n if cid in EOS_TRAFFIC_BYTES_PER_PARTICLE
+                     else None},
         "cpu_baseline": None,
     }
     if not args.no_cpu:
