@@ -71,8 +71,17 @@ const char* nnet_last_error(const nnet_ctx* ctx);
 
 /* Load (or replace) the Helmholtz table: 21 arrays of 541x201 doubles in the reference order
  * f, fd, ft, fdd, ftt, fdt, fddt, fdtt, fddtt, dpdf, dpdfd, dpdft, dpdfdt, ef, efd, eft, efdt,
- * xf, xfd, xft, xfdt, each [density][temperature] (helmholtz.cpp:100-137). */
+ * xf, xfd, xft, xfdt, each [density][temperature] (helmholtz.cpp:100-137), as a binary file of
+ * exactly that content or as the reference's text table (see nnet_helm_table_read). */
 int nnet_ctx_load_helmholtz(nnet_ctx* ctx, const char* path);
+
+/* Read a Helmholtz table file into host memory `out` (21 x 541 x 201 doubles in the layout
+ * above), without a device: the binary layout, or the reference's text table -- for each
+ * temperature j and density i the nine f values, then the dpdf, ef and xf blocks of four,
+ * whitespace-separated, optionally wrapped in STR( ... ) -- exactly as readCPUTable parses it
+ * (helmholtz.cpp:89-137).  nnet_ctx_load_helmholtz accepts the same two formats.  Returns
+ * NNET_E_TABLE for a missing, short or malformed file. */
+int nnet_helm_table_read(const char* path, double* out);
 
 /* Network = reaction list + rate parameterization.  The list uses the reference's flat
  * ReactionList layout (nuclear_net.hpp:168-213): reaction r has reactants

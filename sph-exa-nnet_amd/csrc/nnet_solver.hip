@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1899,16 +1901,76 @@ extern "C"
         return NNET_OK;
     }
 
+    int nnet_helm_table_read(const char* path, double* out)
+    {
+        if (!path || !out) return NNET_E_ARG;
+        const size_t cells = (size_t)HELM_IMAX * HELM_JMAX, total = 21 * cells;
+        FILE*        f     = std::fopen(path, "rb");
+        if (!f) return NNET_E_TABLE;
+        std::string buf;
+        {
+            char   chunk[1 << 16];
+            size_t got;
+            while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0)
+                buf.append(chunk, got);
+        }
+        std::fclose(f);
+        // the binary layout: 21 arrays [density][temperature], nothing else
+        if (buf.size() == total * sizeof(double))
+        {
+            std::memcpy(out, buf.data(), buf.size());
+            return NNET_OK;
+        }
+        // the reference's text table (helmholtz.cpp:89-137): per temperature j and density i the
+        // nine free-energy values, then the dpdf, ef and xf blocks of four, whitespace-separated;
+        // optionally wrapped in STR( ... ) as the reference #includes it (helmholtz.cpp:34-35, 92-94)
+        const char* q   = buf.c_str();
+        const char* end = q + buf.size();
+        while (q < end && std::isspace((unsigned char)*q))
+            ++q;
+        if (end - q >= 4 && std::strncmp(q, "STR(", 4) == 0)
+        {
+            q += 4;
+            while (end > q && std::isspace((unsigned char)end[-1]))
+                --end;
+            if (end == q || end[-1] != ')') return NNET_E_TABLE;
+            --end;
+        }
+        const size_t q_off = (size_t)(q - buf.c_str());
+        buf.resize((size_t)(end - buf.c_str())); // the number stream ends at the closing parenthesis
+        q = buf.c_str() + q_off;
+        auto next = [&](double& v) -> bool
+        {
+            char* e = nullptr;
+            v       = std::strtod(q, &e);
+            if (e == q) return false;
+            q = e;
+            return true;
+        };
+        auto at = [&](int k, int i, int j) -> double& { return out[(size_t)k * cells + (size_t)i * HELM_JMAX + j]; };
+        for (int j = 0; j < HELM_JMAX; ++j)
+            for (int i = 0; i < HELM_IMAX; ++i)
+                for (int k = 0; k < 9; ++k)
+                    if (!next(at(k, i, j))) return NNET_E_TABLE;
+        for (int blk = 0; blk < 3; ++blk)
+            for (int j = 0; j < HELM_JMAX; ++j)
+                for (int i = 0; i < HELM_IMAX; ++i)
+                    for (int k = 0; k < 4; ++k)
+                        if (!next(at(9 + 4 * blk + k, i, j))) return NNET_E_TABLE;
+        while (*q && std::isspace((unsigned char)*q))
+            ++q;
+        return *q ? NNET_E_TABLE : NNET_OK; // nothing may follow the last value
+    }
+
     int nnet_ctx_load_helmholtz(nnet_ctx* c, const char* path)
     {
         if (!c || !path) return NNET_E_ARG;
-        FILE* f = std::fopen(path, "rb");
-        if (!f) return fail(c, NNET_E_TABLE, std::string("cannot open Helmholtz table ") + path);
         const size_t        cells = (size_t)HELM_IMAX * HELM_JMAX;
         std::vector<double> arr((size_t)21 * cells);
-        const size_t        got = std::fread(arr.data(), sizeof(double), arr.size(), f);
-        std::fclose(f);
-        if (got != arr.size()) return fail(c, NNET_E_TABLE, "Helmholtz table too short");
+        const int           rc = nnet_helm_table_read(path, arr.data());
+        if (rc != NNET_OK)
+            return fail(c, rc, std::string("cannot read Helmholtz table ") + path +
+                                   " (binary 21 x 541 x 201 doubles, or the reference's text table)");
         // grid (helmholtz.cpp:101-169)
         HelmGrid g{};
         for (int i = 0; i < HELM_IMAX; ++i)

@@ -71,6 +71,7 @@ def lib():
         eos_args = [vp, C.c_size_t, C.c_int, dp, vp, vp, C.POINTER(vp), vp, vp, vp, vp, vp]
         L.nnet_helm_eos.argtypes = eos_args + [vp]
         L.nnet_helm_eos_sync.argtypes = eos_args
+        L.nnet_helm_table_read.argtypes = [C.c_char_p, dp]
         _lib = L
     return _lib
 
@@ -79,7 +80,19 @@ EXPORTS = [
     "nnet_ctx_create", "nnet_ctx_destroy", "nnet_last_error", "nnet_ctx_load_helmholtz",
     "nnet_network_create", "nnet_network_builtin", "nnet_network_destroy", "nnet_network_reactions",
     "nnet_compute", "nnet_compute_sync", "nnet_ctx_set_max_iterations", "nnet_helm_eos", "nnet_helm_eos_sync",
+    "nnet_helm_table_read",
 ]
+
+
+def helm_table_read(path):
+    """The 21 Helmholtz table arrays (21 x 541 x 201) of a binary or reference-text table file,
+    parsed by the library on the host (no device needed)."""
+    import numpy as np
+    out = np.empty((21, 541, 201))
+    rc = lib().nnet_helm_table_read(path.encode(), out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise NnetError(f"nnet_helm_table_read({path}): {rc}")
+    return out
 
 
 class Context:

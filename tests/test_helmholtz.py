@@ -186,3 +186,33 @@ def test_helm_eos_batch_is_the_pointwise_eos(oracle_lib):
         e = oracle_lib.eos(1, cfg["Y"][:, i], cfg["temp"][i], cfg["rho"][i], Z=cfg["Z"])
         # oracle_eos order: cv, dpdT, p, cp, c, u, ...
         assert np.array_equal(out[:, i], [e[5], e[0], e[2], e[4], e[1]], equal_nan=True)
+
+
+TEXT_TABLE = os.path.join(ROOT, "sph-exa-nnet_amd", "data", "helm_table.dat")
+
+
+@pytest.mark.skipif(not os.path.exists(TEXT_TABLE) or not os.path.exists(TABLE), reason="tables not generated")
+def test_library_reads_the_reference_text_table(tmp_path):
+    """nnet_helm_table_read (host code of the library, no device) parses the reference's text
+    table -- STR( ... )-wrapped as the reference #includes it, or bare -- into exactly the binary
+    table's values, and rejects a truncated or over-long file."""
+    import nnet_mi355x as nm
+    want = np.fromfile(TABLE, dtype=np.float64).reshape(21, 541, 201)
+    assert np.array_equal(nm.helm_table_read(TABLE).view(np.int64), want.view(np.int64))
+    got = nm.helm_table_read(TEXT_TABLE)
+    assert np.array_equal(got.view(np.int64), want.view(np.int64))
+    text = open(TEXT_TABLE).read()
+    body = text[text.index("STR(") + 4:text.rindex(")")]
+    bare = tmp_path / "bare.dat"
+    bare.write_text(body)
+    assert np.array_equal(nm.helm_table_read(str(bare)).view(np.int64), want.view(np.int64))
+    short = tmp_path / "short.dat"
+    short.write_text(body[:len(body) // 2])
+    with pytest.raises(nm.NnetError):
+        nm.helm_table_read(str(short))
+    extra = tmp_path / "extra.dat"
+    extra.write_text(body + " 1.0\n")
+    with pytest.raises(nm.NnetError):
+        nm.helm_table_read(str(extra))
+    with pytest.raises(nm.NnetError):
+        nm.helm_table_read(str(tmp_path / "missing.dat"))
