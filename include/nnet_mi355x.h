@@ -131,11 +131,21 @@ typedef struct nnet_batch {
                               keeps the particles whose FP32 factors did not refine on FP64. */
 } nnet_batch;
 
+/* use_drhodt of nnet_compute: how rho_m1 (the previous hydro step's density) enters */
+#define NNET_DRHODT_OFF 0       /* not used (drho/dt = 0) */
+#define NNET_DRHODT_CPU 1       /* the reference CPU path (parallel_nuclear_net_impl.cpp:89-92):
+                                   rho_m1[i] (if nonzero) is overwritten by (rho - rho_m1)/previous_dt
+                                   and the integration still uses drho/dt = 0 */
+#define NNET_DRHODT_INTEGRATE 2 /* the reference GPU branch (parallel_nuclear_net.cu:79-81):
+                                   drho/dt = (rho - rho_m1)/previous_dt (0 where rho_m1 is 0), used
+                                   in the integration, rho(t) = rho - drho/dt (hydro_dt - elapsed)
+                                   (nuclear_net.hpp:859); rho_m1 is left unchanged */
+
 /* computeNuclearReactions for nb batches (different networks may share one call).
  * Stream-ordered on `stream` (hipStream_t, NULL = default stream); returns after enqueueing.
  * Semantics follow the reference CPU path (parallel_nuclear_net_impl.cpp:63-103): particles with
- * rho <= 1e5 or T <= 1e8 are skipped; with use_drhodt, rho_m1[i] (if nonzero) is overwritten by
- * (rho - rho_m1)/previous_dt and the integration uses drho/dt = 0. */
+ * rho <= 1e5 or T <= 1e8 are skipped; use_drhodt is one of NNET_DRHODT_* (as in the reference,
+ * the division by previous_dt happens only where rho_m1 is nonzero). */
 int nnet_compute(nnet_ctx* ctx, const nnet_batch* batches, int nb, double hydro_dt,
                  double previous_dt, int use_drhodt, void* stream);
 

@@ -683,11 +683,19 @@ extern "C"
                 {
                     for (int j = 0; j < d; ++j)
                         Yl[j] = Y[j][i];
-                    // parallel_nuclear_net_impl.cpp:90-92: the CPU path overwrites rho_m1 with
-                    // drho/dt and always integrates with drho_dt = 0.
+                    // use_drhodt 1: parallel_nuclear_net_impl.cpp:90-92, the CPU path overwrites
+                    // rho_m1 with drho/dt and integrates with drho_dt = 0.  use_drhodt 2:
+                    // parallel_nuclear_net.cu:79-81, drho/dt from rho_m1 (left unchanged) drives the
+                    // integration.
                     double drho_dt = 0;
-                    if (use_drhodt)
+                    if (use_drhodt == 1)
+                    {
                         if (rho_m1[i] != 0) rho_m1[i] = (rho[i] - rho_m1[i]) / previous_dt;
+                    }
+                    else if (use_drhodt == 2)
+                    {
+                        if (rho_m1[i] != 0) drho_dt = (rho[i] - rho_m1[i]) / previous_dt;
+                    }
                     st = solve_particle(net, d, e, w, Yl.data(), temp[i], rho[i], drho_dt, hydro_dt, dt[i], max_iter,
                                         nullptr);
                     for (int j = 0; j < d; ++j)

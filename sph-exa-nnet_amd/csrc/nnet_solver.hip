@@ -590,10 +590,19 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             }
             continue;
         }
-        // parallel_nuclear_net_impl.cpp:89-92 (the CPU path integrates with drho/dt = 0)
-        if (tid == 0 && sp.use_drhodt)
-            if (bd.rho_m1[p] != 0) bd.rho_m1[p] = (rho - bd.rho_m1[p]) / sp.previous_dt;
-        const double drho_dt = 0;
+        // NNET_DRHODT_CPU: parallel_nuclear_net_impl.cpp:89-92 (rho_m1 is overwritten, the
+        // integration uses drho/dt = 0); NNET_DRHODT_INTEGRATE: parallel_nuclear_net.cu:79-81 (drho/dt
+        // from rho_m1, which is left as it is, drives rho(t), nuclear_net.hpp:859)
+        double drho_dt = 0;
+        if (sp.use_drhodt == NNET_DRHODT_CPU)
+        {
+            if (tid == 0 && bd.rho_m1[p] != 0) bd.rho_m1[p] = (rho - bd.rho_m1[p]) / sp.previous_dt;
+        }
+        else if (sp.use_drhodt == NNET_DRHODT_INTEGRATE)
+        {
+            const double rm1 = bd.rho_m1[p];
+            if (rm1 != 0) drho_dt = (rho - rm1) / sp.previous_dt;
+        }
 
         for (int s = tid; s < D; s += NT)
             sY[s] = bd.Y[s][p];
@@ -2091,6 +2100,8 @@ extern "C"
     {
         if (!c || (nb > 0 && !batches) || nb < 0) return NNET_E_ARG;
         if (!(hydro_dt > 0)) return fail(c, NNET_E_ARG, "hydro_dt must be positive");
+        if (use_drhodt != NNET_DRHODT_OFF && use_drhodt != NNET_DRHODT_CPU && use_drhodt != NNET_DRHODT_INTEGRATE)
+            return fail(c, NNET_E_ARG, "use_drhodt must be NNET_DRHODT_OFF, _CPU or _INTEGRATE");
         HIP_TRY(c, hipSetDevice(c->device));
         hipStream_t st = (hipStream_t)stream;
         StepParams  sp{hydro_dt, previous_dt, use_drhodt, c->max_iter};
@@ -2151,7 +2162,7 @@ extern "C"
             if (!b.network || b.network->ctx != c) return fail(c, NNET_E_ARG, "batch without a network of this context");
             if (b.n == 0) continue;
             ++n_live;
-            if (!b.rho || !b.Y || !b.temp || !b.dt || (use_drhodt && !b.rho_m1))
+            if (!b.rho || !b.Y || !b.temp || !b.dt || (use_drhodt != NNET_DRHODT_OFF && !b.rho_m1))
                 return fail(c, NNET_E_ARG, "null particle array");
             if (b.precision != NNET_PRECISION_FP64 && b.precision != NNET_PRECISION_MIXED)
                 return fail(c, NNET_E_ARG, "unknown precision");
@@ -2258,7 +2269,7 @@ extern "C"
             redo_bd.qrange      = ctr + 3; // {0, redo count}
             redo_bd.fp64_pass   = 1;
             StepParams redo_sp  = sp;
-            redo_sp.use_drhodt  = 0;
+            if (sp.use_drhodt == NNET_DRHODT_CPU) redo_sp.use_drhodt = NNET_DRHODT_OFF; // rho_m1 already updated
             auto dispatch       = [&](auto netc, auto eosc)
             {
                 constexpr int NV = decltype(netc)::value, EV = decltype(eosc)::value;

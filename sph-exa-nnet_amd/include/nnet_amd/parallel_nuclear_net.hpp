@@ -286,6 +286,10 @@ struct Options
     uint32_t* fp64_solves = nullptr; // device (optional, mixed precision)
     const uint32_t* cost_hint = nullptr; // device (optional): expected cost per particle, e.g. newton of the last call
     const uint32_t* fp64_hint = nullptr; // device (optional, mixed precision): nonzero -> FP64 path, e.g. fp64_solves of the last call
+    // what use_drhodt = true means: NNET_DRHODT_CPU (the reference CPU path, rho_m1 overwritten by
+    // drho/dt, integration with drho/dt = 0) or NNET_DRHODT_INTEGRATE (the reference GPU branch:
+    // drho/dt drives rho(t), rho_m1 unchanged)
+    int drhodt_mode = NNET_DRHODT_CPU;
 };
 
 template<bool use_gpu, class Data, typename Float, class List, int KIND, class Eos, class nseFunction = void*>
@@ -338,7 +342,8 @@ void computeNuclearReactions(Data& n, size_t firstIndex, size_t lastIndex, const
         b.z_dim = (int)eos.Z.size();
     }
     if (use_drhodt && b.rho_m1 == nullptr) use_drhodt = false; // the reference disables it with a warning
-    detail::check(c, nnet_compute_sync(c, &b, 1, hydro_dt, previous_dt, use_drhodt ? 1 : 0), "nnet_compute");
+    detail::check(c, nnet_compute_sync(c, &b, 1, hydro_dt, previous_dt, use_drhodt ? opt.drhodt_mode : NNET_DRHODT_OFF),
+                  "nnet_compute");
 }
 
 /* Post-step EOS refresh, mirroring

@@ -14,6 +14,7 @@ LIB_PATH = os.environ.get("NNET_MI355X_LIB") or os.path.join(HERE, "lib", "libnn
 RATES = {14: 14, 86: 86, 87: 87}
 EOS_IDEAL, EOS_HELMHOLTZ = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+DRHODT_OFF, DRHODT_CPU, DRHODT_INTEGRATE = 0, 1, 2
 STATUS = {0: "OK", 1: "SKIPPED", 2: "STALLED", 3: "ZERO_DT", 4: "MAX_ITER"}
 N_SPECIES = {14: 14, 86: 86, 87: 87}
 MAXSP = 88
@@ -167,7 +168,8 @@ class Context:
         b._ykeep = yp
         return b
 
-    def compute(self, batches, hydro_dt, previous_dt=None, use_drhodt=False, stream=None, sync=True):
+    def compute(self, batches, hydro_dt, previous_dt=None, use_drhodt=DRHODT_OFF, stream=None, sync=True):
+        """use_drhodt: DRHODT_OFF / DRHODT_CPU / DRHODT_INTEGRATE (True means DRHODT_CPU)."""
         arr = (Batch * len(batches))(*batches)
         prev = hydro_dt if previous_dt is None else previous_dt
         if sync and stream is None:

@@ -44,6 +44,8 @@ def test_binding_constants_match_header():
     mac = header_macros()
     assert nm.PRECISION_FP64 == mac["NNET_PRECISION_FP64"]
     assert nm.PRECISION_MIXED == mac["NNET_PRECISION_MIXED"]
+    assert (nm.DRHODT_OFF, nm.DRHODT_CPU, nm.DRHODT_INTEGRATE) == (
+        mac["NNET_DRHODT_OFF"], mac["NNET_DRHODT_CPU"], mac["NNET_DRHODT_INTEGRATE"])
     for name in ("NNET_STATUS_OK", "NNET_STATUS_SKIPPED", "NNET_STATUS_STALLED", "NNET_STATUS_ZERO_DT",
                  "NNET_STATUS_MAX_ITER"):
         assert name in mac

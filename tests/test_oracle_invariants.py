@@ -140,3 +140,29 @@ def test_dynamic_batch_size(oracle_lib):
     assert oracle_lib.L.oracle_dynamic_batch_size(1000, 8) == 3
     # N/P = 2^17: f = floor(17 / phi) = 10 -> floor(2^21 / (2^10 * 2 * 16)) = 64
     assert oracle_lib.L.oracle_dynamic_batch_size(2097152, 16) == 64
+
+
+def test_batch_drhodt_modes(oracle_lib):
+    """oracle_batch: use_drhodt 1 overwrites rho_m1 and integrates with drho/dt = 0 (the CPU path);
+    use_drhodt 2 leaves rho_m1 and integrates each particle exactly as the single-particle path with
+    drho/dt = (rho - rho_m1)/previous_dt."""
+    cfg = workloads.config(1, n=60, seed=3)
+    rng = np.random.default_rng(3)
+    rm1 = cfg["rho"] * (1 + rng.uniform(-0.2, 0.2, 60))
+    rm1[::7] = 0.0
+    prev = 2e-4
+    res = {}
+    for mode in (0, 1, 2):
+        Y, T, dt, r = np.ascontiguousarray(cfg["Y"].copy()), cfg["temp"].copy(), cfg["dt"].copy(), rm1.copy()
+        st, _, _ = oracle_lib.batch(14, cfg["rho"].copy(), T, Y, dt, 1e-3, previous_dt=prev, rho_m1=r,
+                                    use_drhodt=mode, max_iter=3000)
+        res[mode] = (Y, T, r, st)
+    assert np.array_equal(res[1][0], res[0][0]) and np.array_equal(res[1][1], res[0][1])
+    live = (cfg["rho"] > 1e5) & (cfg["temp"] > 1e8) & (rm1 != 0)
+    assert np.array_equal(res[1][2][live], (cfg["rho"][live] - rm1[live]) / prev)
+    assert np.array_equal(res[2][2], rm1)
+    for k in np.nonzero(live)[0][:12]:
+        d = (cfg["rho"][k] - rm1[k]) / prev
+        a = oracle_lib.particle(14, cfg["Y"][:, k], cfg["temp"][k], cfg["rho"][k], 1e-3, 1e-5, drho_dt=d,
+                                max_iter=3000)
+        assert np.array_equal(a[0], res[2][0][:, k]) and a[1] == res[2][1][k], k

@@ -54,3 +54,31 @@ def test_reference_solve_substep_agrees(oracle_lib, ref_lib):
         assert same(Y, a[0]) and T[0] == a[1] and dt[0] == a[2]
         checked += 1
     assert checked > 10
+
+
+@pytest.mark.parametrize("net,eos", [(14, 0), (86, 1)])
+def test_particles_with_drho_dt(oracle_lib, ref_lib, net, eos):
+    """A nonzero drho/dt (rho(t) = rho - drho/dt (dt_tot - elapsed), nuclear_net.hpp:859; the
+    dpdT drho/rho^2 terms of the Newton matrix, :603-608): the oracle's particle path against the
+    reference's prepareSystemSubstep / finalizeSystemSubstep loop, bit for bit."""
+    import os
+    if eos == 1:
+        if not ref_lib.has_helmholtz:
+            pytest.skip("reference built without a Helmholtz table")
+        table = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sph-exa-nnet_amd",
+                             "data", "helm_table.bin")
+        oracle_lib.helm_load(table)
+    cfg = workloads.config(1 if net == 14 else 3, n=40, seed=17)
+    rng = np.random.default_rng(net)
+    checked = 0
+    for k in range(40):
+        if not (cfg["rho"][k] > 1e5 and cfg["temp"][k] > 1e8):
+            continue
+        drho_dt = cfg["rho"][k] * rng.uniform(-300, 300)
+        kw = dict(eos_kind=eos, mu=10.0, Z=cfg["Z"], trace_cap=2000, drho_dt=drho_dt, max_iter=3000)
+        a = oracle_lib.particle(cfg["net"], cfg["Y"][:, k], cfg["temp"][k], cfg["rho"][k], 1e-4, 1e-5, **kw)
+        b = ref_lib.particle(cfg["net"], cfg["Y"][:, k], cfg["temp"][k], cfg["rho"][k], 1e-4, 1e-5, **kw)
+        assert same(a[0], b[0]) and a[1] == b[1] and a[2] == b[2], k
+        assert np.array_equal(a[3][:4], b[3][:4]) and same(a[4], b[4]), k
+        checked += 1
+    assert checked >= 8
