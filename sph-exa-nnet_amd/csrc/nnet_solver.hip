@@ -88,7 +88,7 @@ struct NetDev
     //   row_ptr16[d+1], nz_col8[nnz]: non-zeros of row a are [row_ptr[a], row_ptr[a+1]), columns
     //                 ascending
     const uint32_t* img;
-    int             img_words, oRx, oEnt16, oSpPtr16, oSpEnt, oNzPtr16, oNzEnt, oRowPtr16, oRowEnt;
+    int             img_words, oRx, oEnt16, oSpPtr16, oSpEnt, oNzPtr16, oNzEnt, oRowPtr16, oRowEnt, oJord16;
     // per lane, per register slot: index of the Jacobian non-zero the slot holds (-1: zero),
     // laid out [slot][lane] for coalesced loads
     const uint4*   lanemap; // [lane]: register-slot mask (x, y), first non-zero id (z)
@@ -161,17 +161,17 @@ struct Cfg;
 template<>
 struct Cfg<14>
 {
-    static constexpr int D = 14, N = 15, NR = 29, NE = 16, NT = 64, TT = 1, WR = 1, NNZ = 72, IMG = 512, WPE = 4;
+    static constexpr int D = 14, N = 15, NR = 29, NE = 16, NT = 64, TT = 1, WR = 1, NNZ = 72, IMG = 576, WPE = 4;
 };
 template<>
 struct Cfg<86>
 {
-    static constexpr int D = 86, N = 87, NR = 309, NE = 157, NT = 256, TT = 6, WR = 2, NNZ = 800, IMG = 4736, WPE = 2;
+    static constexpr int D = 86, N = 87, NR = 309, NE = 157, NT = 256, TT = 6, WR = 2, NNZ = 800, IMG = 5248, WPE = 2;
 };
 template<>
 struct Cfg<87>
 {
-    static constexpr int D = 87, N = 88, NR = 311, NE = 157, NT = 256, TT = 6, WR = 2, NNZ = 800, IMG = 4736, WPE = 2;
+    static constexpr int D = 87, N = 88, NR = 311, NE = 157, NT = 256, TT = 6, WR = 2, NNZ = 800, IMG = 5248, WPE = 2;
 };
 
 template<int NET, bool MIX>
@@ -504,8 +504,10 @@ struct TileLU
 template<int NET>
 __device__ __forceinline__ int special_task(int tid)
 {
-    // net86/net87: two groups on lanes 32, 33 of wavefronts 0, 1, 2 (next to the fitted rates)
-    if (NET != 14) return (tid & 63) >= 32 && (tid & 63) < 34 && tid < 192 ? (tid >> 6) * 2 + ((tid & 63) - 32) : -1;
+    // net86/net87: groups 0-3 on lanes 32, 33 of wavefronts 0 and 1 (next to the fitted rates),
+    // group 4 on wavefront 2 (fitted rates + species), group 5 on wavefront 3 (species + EOS): the
+    // placement that measured the shortest rate phase (its slowest wavefront sets its length)
+    if (NET != 14) return (tid == 32 || tid == 33) ? tid - 32 : (tid == 96 || tid == 97) ? tid - 94 : tid == 160 ? 4 : tid == 233 ? 5 : -1;
     return tid >= 32 && tid < 38 ? tid - 32 : -1;
 }
 
@@ -562,6 +564,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
     const uint32_t* spent  = simg + nd.oSpEnt;
     const uint16_t* nzp16  = reinterpret_cast<const uint16_t*>(simg) + nd.oNzPtr16;
     const uint32_t* nzent  = simg + nd.oNzEnt;
+    const uint16_t* jord16 = reinterpret_cast<const uint16_t*>(simg) + nd.oJord16;
     const uint16_t* rowp16 = reinterpret_cast<const uint16_t*>(simg) + nd.oRowPtr16;
     const uint32_t* rowent = simg + nd.oRowEnt;
 
@@ -883,8 +886,11 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
 
             // ---- Jacobian non-zeros (firstOrderDYFromReactions) and species sums
             //      (derivativesFromReactions), each summed in the reference's order ----
-            for (int e = tid; e < nd.nnz; e += NT)
+            for (int k = tid; k < nd.nnz; k += NT)
             {
+                // non-zeros are handed out in descending order of their list lengths, so the
+                // lanes of one pass over the workgroup have lists of similar length
+                const int e = jord16[k];
                 // J = sum over the entry's contributions in list order of count * mass-action
                 // derivative, skipping reactions whose derivative factor is <= 1e-40; eight
                 // contributions are loaded ahead of their (sequential) additions
@@ -1803,6 +1809,21 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
     push_half(h);
     const int oRowEnt = (int)img.size();
     img.insert(img.end(), rowent.begin(), rowent.end());
+    // compute order of the non-zeros: descending cost (8-wide load groups + serial remainder)
+    const int oJord16 = (int)img.size() * 2;
+    {
+        std::vector<int> cost(nnz);
+        for (int e = 0; e < nnz; ++e)
+        {
+            const int len = (int)ent[(size_t)nz_row[e] * d + nz_col[e]].size();
+            cost[e]       = len / 8 + len % 8;
+        }
+        h.clear();
+        for (int e = 0; e < nnz; ++e)
+            h.push_back((uint16_t)e);
+        std::stable_sort(h.begin(), h.end(), [&](uint16_t a, uint16_t b) { return cost[a] > cost[b]; });
+        push_half(h);
+    }
 
     const int img_cap = kind == 14 ? Cfg<14>::IMG : Cfg<86>::IMG;
     const int nnz_cap = kind == 14 ? Cfg<14>::NNZ : Cfg<86>::NNZ;
@@ -1832,7 +1853,7 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
     n.etab = (double*)(B + oElTab), n.BEe = NNET87_ELECTRON_BE_MEV[0] * NNET_MEV_TO_ERG;
     n.img = (uint32_t*)(B + oImg), n.img_words = (int)img.size(), n.oRx = oRx, n.oEnt16 = oEnt16;
     n.oSpPtr16 = oSpPtr16, n.oSpEnt = oSpEnt, n.oNzPtr16 = oNzPtr16, n.oNzEnt = oNzEnt, n.oRowPtr16 = oRowPtr16;
-    n.oRowEnt = oRowEnt, n.lanemap = (const uint4*)(B + oLane);
+    n.oRowEnt = oRowEnt, n.oJord16 = oJord16, n.lanemap = (const uint4*)(B + oLane);
     return NNET_OK;
 }
 
