@@ -469,32 +469,79 @@ struct TileLU
     // row above it (column order, j descending); returns this lane's rows in acc
     static __device__ __forceinline__ void back(const R* sU, R (&acc)[(N + 63) / 64], int lane)
     {
+        // the U entries of four columns are loaded ahead of their four dependent steps (a
+        // scheduling barrier keeps each group's loads together), so the v_readlane/FMA chain does
+        // not wait for LDS at every step (groups of 4 measured faster than 8 and 16: fewer live
+        // registers)
+        constexpr int NA = (N + 63) / 64, G = 4;
 #pragma unroll
-        for (int j = N - 1; j > 0; --j)
+        for (int j0 = N - 1; j0 > 0; j0 -= G)
         {
-            const R xj = bcast_lane(acc[j >> 6], j & 63);
+            R u[G][NA];
 #pragma unroll
-            for (int a = 0; a <= (j - 1) >> 6; ++a)
+            for (int s = 0; s < G; ++s)
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+                {
+                    const int j = j0 - s, row = lane + 64 * a;
+                    u[s][a]     = (j > 0 && row < j) ? sU[urow(row) + j] : R(0);
+                }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int s = 0; s < G; ++s)
             {
-                const int row = lane + 64 * a;
-                if (row < j) acc[a] = fma(-sU[urow(row) + j], xj, acc[a]);
+                const int j = j0 - s;
+                if (j > 0)
+                {
+                    const R xj = bcast_lane(acc[j >> 6], j & 63);
+#pragma unroll
+                    for (int a = 0; a <= (j - 1) >> 6; ++a)
+                    {
+                        const int row = lane + 64 * a;
+                        if (row < j) acc[a] = fma(-u[s][a], xj, acc[a]);
+                    }
+                }
             }
         }
     }
     // y = forward elimination of r with the stored multipliers (eigen.hpp:182-199 applied to a new
-    // right-hand side): y_j = (r_j - sum_{k<j} L_jk y_k) / d_j, in column order
+    // right-hand side): y_j = (r_j - sum_{k<j} L_jk y_k) / d_j, in column order; loads staged as
+    // in back()
     static __device__ __forceinline__ void forward(const R* sL, const R* sRinv, R (&acc)[(N + 63) / 64], int lane)
     {
+        constexpr int NA = (N + 63) / 64, G = 4;
 #pragma unroll
-        for (int j = 0; j < N; ++j)
+        for (int j0 = 0; j0 < N; j0 += G)
         {
-            const R yj = bcast_lane(acc[j >> 6], j & 63) * sRinv[j];
-            if ((j & 63) == lane) acc[j >> 6] = yj;
+            R l[G][NA], ri[G];
 #pragma unroll
-            for (int a = (j + 1) >> 6; a < (N + 63) / 64; ++a)
+            for (int s = 0; s < G; ++s)
             {
-                const int row = lane + 64 * a;
-                if (row > j && row < N) acc[a] = fma(-sL[lrow(row) + j], yj, acc[a]);
+                const int j = j0 + s;
+                ri[s]       = j < N ? sRinv[j] : R(0);
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+                {
+                    const int row = lane + 64 * a;
+                    l[s][a]       = (j < N && row > j && row < N) ? sL[lrow(row) + j] : R(0);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int s = 0; s < G; ++s)
+            {
+                const int j = j0 + s;
+                if (j < N)
+                {
+                    const R yj = bcast_lane(acc[j >> 6], j & 63) * ri[s];
+                    if ((j & 63) == lane) acc[j >> 6] = yj;
+#pragma unroll
+                    for (int a = (j + 1) >> 6; a < NA; ++a)
+                    {
+                        const int row = lane + 64 * a;
+                        if (row > j && row < N) acc[a] = fma(-l[s][a], yj, acc[a]);
+                    }
+                }
             }
         }
     }
