@@ -319,6 +319,11 @@ struct TileLU
         constexpr int NT = C::NT, HALF = NT / 2;
         static_assert(HALF >= N - 3, "one lane per trailing row and per trailing column");
         const R eps = (R)k::epsilonSystem;
+        // opaque copies of the lane coordinates: the block's lane-dependent LDS addresses are
+        // formed here, not hoisted to the kernel entry for all 22 blocks at once (which keeps
+        // dozens of address registers live across the factorization and spills them); the
+        // four-block net14 factorization does not need it
+        if constexpr (NET != 14) asm volatile("" : "+v"(tid), "+v"(lr), "+v"(lc));
         R*      P   = pan + (STEP & 1) * 8 * NPAD;
         R*      RT  = P + 4 * NPAD;
 
