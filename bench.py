@@ -242,7 +242,7 @@ def main():
     ap.add_argument("--workload", default="solve", choices=["solve", "helm-eos"],
                     help="solve (default): computeNuclearReactions; helm-eos: the post-step EOS refresh "
                          "computeHelmEOS over the same particle set")
-    ap.add_argument("--cpu-sample", type=int, default=262144,
+    ap.add_argument("--cpu-sample", type=int, default=524288,
                     help="particles of the CPU-baseline sample (the oracle on all host cores, ~10-30 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

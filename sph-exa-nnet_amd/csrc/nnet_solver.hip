@@ -474,6 +474,14 @@ struct TileLU
     // row above it (column order, j descending); returns this lane's rows in acc
     static __device__ __forceinline__ void back(const R* sU, R (&acc)[(N + 63) / 64], int lane)
     {
+        // an opaque lane index (range restored): the addresses of this solve are formed here
+        // rather than kept live from the kernel entry (the mixed kernel's refinement calls it
+        // up to four times per iteration)
+        if constexpr (NET != 14)
+        {
+            asm volatile("" : "+v"(lane));
+            lane &= 63;
+        }
         // the U entries of four columns are loaded ahead of their four dependent steps (a
         // scheduling barrier keeps each group's loads together), so the v_readlane/FMA chain does
         // not wait for LDS at every step (groups of 4 measured faster than 8 and 16: fewer live
@@ -514,6 +522,11 @@ struct TileLU
     // in back()
     static __device__ __forceinline__ void forward(const R* sL, const R* sRinv, R (&acc)[(N + 63) / 64], int lane)
     {
+        if constexpr (NET != 14)
+        {
+            asm volatile("" : "+v"(lane)); // as in back()
+            lane &= 63;
+        }
         constexpr int NA = (N + 63) / 64, G = 4;
 #pragma unroll
         for (int j0 = 0; j0 < N; j0 += G)
