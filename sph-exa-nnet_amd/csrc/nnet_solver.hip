@@ -156,12 +156,15 @@ __device__ __forceinline__ double factorial_count(int n) { return n == 2 ? 2. : 
 // ---------------------------------------------------------------------------------------------
 // Kernel configuration per network
 // ---------------------------------------------------------------------------------------------
+// NT threads per workgroup (one particle), TT x TT tiles of 16 on a WR x WR wave grid, NNZ / IMG
+// capacities of the Jacobian pattern and of the LDS assembly image, WPE = waves per SIMD the
+// register budget is sized for (net14: 3 measured faster than 4 (spills) and 2 (fewer particles))
 template<int NET>
 struct Cfg;
 template<>
 struct Cfg<14>
 {
-    static constexpr int D = 14, N = 15, NR = 29, NE = 16, NT = 64, TT = 1, WR = 1, NNZ = 72, IMG = 576, WPE = 4;
+    static constexpr int D = 14, N = 15, NR = 29, NE = 16, NT = 64, TT = 1, WR = 1, NNZ = 72, IMG = 576, WPE = 3;
 };
 template<>
 struct Cfg<86>
