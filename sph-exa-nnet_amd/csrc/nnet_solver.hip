@@ -568,14 +568,16 @@ struct TileLU
     }
 };
 
-// Lane of the first special-rate group task / of the electron-capture task.
+// Hand-fitted rate group computed by lane tid (-1: none).
 template<int NET>
 __device__ __forceinline__ int special_task(int tid)
 {
     // net86/net87: groups 0-3 on lanes 32, 33 of wavefronts 0 and 1 (next to the fitted rates),
-    // group 4 on wavefront 2 (fitted rates + species), group 5 on wavefront 3 (species + EOS): the
-    // placement that measured the shortest rate phase (its slowest wavefront sets its length)
-    if (NET != 14) return (tid == 32 || tid == 33) ? tid - 32 : (tid == 96 || tid == 97) ? tid - 94 : tid == 160 ? 4 : tid == 233 ? 5 : -1;
+    // group 4 on wavefront 2 (fitted rates + species); group 5 on wavefront 1 for net86 and on
+    // wavefront 3 (species + EOS) for net87, whose wavefront 1 runs the electron captures: the
+    // placements that measured the shortest rate phase (its slowest wavefront sets its length)
+    if (NET == 86) return (tid == 32 || tid == 33) ? tid - 32 : (tid == 96 || tid == 97) ? tid - 94 : tid == 160 ? 4 : tid == 98 ? 5 : -1;
+    if (NET == 87) return (tid == 32 || tid == 33) ? tid - 32 : (tid == 96 || tid == 97) ? tid - 94 : tid == 160 ? 4 : tid == 233 ? 5 : -1;
     return tid >= 32 && tid < 38 ? tid - 32 : -1;
 }
 
@@ -862,7 +864,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                         sDl[nd.sp.roag_l] = drl;
                     }
                 }
-                if (NET == 87 && tid == 224)
+                if (NET == 87 && tid == 98) // wavefront 1: fitted rates only, the lightest
                     electron_interpolate(nd.elogT, nd.elogR, nd.etab, Tg, sYb[86] * rho_t, sEv);
                 if (NET != 14 && tid == 225) sEff[3] = sDeff[3] = sL[3] = sDl[3] = 0; // unused slots
             }
