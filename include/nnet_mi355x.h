@@ -128,7 +128,9 @@ typedef struct nnet_batch {
                               nonzero entry skips the FP32 attempt and is integrated by the FP64
                               pass (its result is then the FP64 path's).  Passing the previous
                               call's fp64_solves (the same array may be passed as fp64_solves)
-                              keeps the particles whose FP32 factors did not refine on FP64. */
+                              keeps the particles whose FP32 factors did not refine on FP64.
+                              Honoured in calls with one non-empty batch outside HIP graph
+                              capture; ignored otherwise. */
 } nnet_batch;
 
 /* use_drhodt of nnet_compute: how rho_m1 (the previous hydro step's density) enters */

@@ -2268,6 +2268,15 @@ extern "C"
                 return fail(c, NNET_E_ARG, "unknown EOS");
         }
         const bool fork = n_live > 1;
+        // the fp64 routing hint forks an auxiliary stream per batch; it is honoured in single-batch
+        // calls outside HIP graph capture and ignored otherwise (results stay valid without it)
+        hipStreamCaptureStatus capture = hipStreamCaptureStatusActive;
+        if (hipStreamIsCapturing(st, &capture) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            capture = hipStreamCaptureStatusActive; // unknown: treat as captured
+        }
+        const bool allow_route = !fork && capture == hipStreamCaptureStatusNone;
         if (fork)
         {
             HIP_TRY(c, hipEventRecord(c->fork_ev, st));
@@ -2311,7 +2320,7 @@ extern "C"
             }
             // ordered dispatch: longest first by the cost hint; in a mixed batch, the particles with
             // a nonzero fp64 hint form the prefix [0, nH) of the order, the rest [nH, n)
-            const uint32_t*     route = mix ? b.fp64_hint : nullptr;
+            const uint32_t*     route = (mix && allow_route) ? b.fp64_hint : nullptr;
             const unsigned int* range = nullptr;
             bd.order                  = nullptr;
             if (b.cost_hint || route)
