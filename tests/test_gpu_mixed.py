@@ -98,3 +98,31 @@ def test_mixed_vs_fp64_oracle(ctx, oracle_lib, cid, n):
                           sp, Y, T, so, max_run_orc=oracle_lib.last_max_run)
         print(cfg["label"], "perturbed oracle vs oracle:", rp, flush=True)
         assert r["n"] - r["within"] <= 2 * (rp["n"] - rp["within"]) + math.ceil(0.01 * r["n"]), (r, rp)
+
+
+def test_hint_ignored_in_multi_batch_call(ctx):
+    """fp64_hint is honoured in single-batch calls only: in a call with two mixed batches a hint that
+    marks every particle changes nothing (same bits and fp64_solves as without a hint)."""
+    import torch
+    import nnet_mi355x as nm
+    cfgs = [workloads.config(2, n=512, seed=61), workloads.config(3, n=256, seed=62)]
+    print("mixed two-batch call with and without fp64_hint", flush=True)
+    runs = []
+    for hint in (False, True):
+        sts, bs = [], []
+        for cfg in cfgs:
+            st = nm.device_state(torch, cfg["net"], cfg["rho"], cfg["temp"], cfg["Y"], cfg["dt"])
+            if hint:
+                st["fp64_hint"] = torch.ones_like(st["fp64_solves"])
+            bs.append(ctx.make_batch(cfg["net"], st, eos=cfg["eos"], mu=cfg["mu"], Z=cfg["Z"],
+                                     precision=nm.PRECISION_MIXED))
+            sts.append(st)
+        ctx.compute(bs, cfgs[0]["hydro_dt"])
+        torch.cuda.synchronize()
+        runs.append([{k: s[k].cpu().numpy() for k in ("Y", "temp", "dt", "status", "newton", "fp64_solves")}
+                     for s in sts])
+    assert all(cfg["hydro_dt"] == cfgs[0]["hydro_dt"] for cfg in cfgs)
+    for a, b in zip(*runs):
+        for k in a:
+            assert np.array_equal(a[k], b[k], equal_nan=True), k
+    assert runs[0][1]["fp64_solves"].sum() < runs[0][1]["newton"].sum()  # the hint would have changed this
