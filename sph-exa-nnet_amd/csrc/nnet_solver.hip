@@ -43,9 +43,12 @@ constexpr double epsilonSystem = 1e-40, epsilonVector = 1e-16;
 constexpr double NR_maxDt = 1e-2, NR_dTOverTempTarget = 1e-2, NR_dTOverTempTol = 4, NR_itTol = 1e-7;
 constexpr int    NR_maxIt      = 11;
 constexpr double substep_dtTol = 1e-6;
-// mixed-precision solve: at most this many FP64-residual refinement sweeps of the FP32 factors,
-// converged when every correction is below mixedTol of its component's scale
-constexpr int    mixedSweeps = 3;
+// mixed-precision solve: at most mixed_sweeps(net) FP64-residual refinement sweeps of the FP32
+// factors, converged when every correction is below mixedTol of its component's scale.  On net86 /
+// net87 an iteration that needs a second sweep mostly fails the third too, so one sweep and an
+// early hand-over to the FP64 pass is faster (net86 Helmholtz, 2M particles: 885k against 825k
+// particle-steps/s with three); net14's 15x15 systems refine within three sweeps.
+constexpr int mixed_sweeps(int net) { return net == 14 ? 3 : 1; }
 constexpr double mixedTol    = 1e-11;
 } // namespace k
 
@@ -1213,7 +1216,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                 __syncthreads();
                 double* sRes = sPan; // the panel buffers are free after the LU
                 bool    conv = false;
-                for (int sweep = 0; sweep < k::mixedSweeps && !conv; ++sweep)
+                for (int sweep = 0; sweep < k::mixed_sweeps(NET) && !conv; ++sweep)
                 {
                     // FP64 residual r = b - M x over the sparse pattern (row 0 is dense)
                     for (int ri = tid; ri < N; ri += NT)
