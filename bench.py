@@ -36,10 +36,10 @@ PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix)
 # copy (`bench.py --copy-calibration 512`: FETCH_SIZE reads 0.5x and WRITE_SIZE 1.0x of the copied
 # bytes).  FP64 (`--precision fp64 --steps 1 --warmup 0 --no-cpu`, one dispatch): 2 x 4,857,641 KiB
 # fetched + 4,010,909 KiB written.  Mixed (`--precision mixed --steps 1 --warmup 1 --no-cpu`, the
-# hinted step: routed FP64 pass + mixed pass + redo pass): 2 x 6,058,217 KiB fetched + 4,258,767
+# hinted step: routed FP64 pass + mixed pass + redo pass): 2 x 6,702,294 KiB fetched + 4,368,279
 # KiB written; per 2,097,152 particles.
 TRAFFIC_BYTES_PER_PARTICLE_STEP = {(3, "fp64"): (2 * 4857640.94 + 4010909.16) * 1024 / 2097152,
-                                   (3, "mixed"): (2 * 6058216.81 + 4258767.31) * 1024 / 2097152}
+                                   (3, "mixed"): (2 * 6702293.56 + 4368279.28) * 1024 / 2097152}
 
 # The same for the EOS refresh kernel over the C3 particle set (`--workload helm-eos --steps 2
 # --warmup 0 --no-cpu`): 2 x 755,378 KiB fetched + 81,920 KiB written per 2,097,152 particles.

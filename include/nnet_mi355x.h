@@ -45,9 +45,9 @@ extern "C" {
 
 /* arithmetic of the Newton linear solve */
 #define NNET_PRECISION_FP64 0  /* plain FP64 everywhere (parity path) */
-#define NNET_PRECISION_MIXED 1 /* FP32 LU on MFMA + FP64 iterative refinement (<= 3 sweeps); a
-                                  particle with an iteration that does not converge is integrated
-                                  by the FP64 path instead */
+#define NNET_PRECISION_MIXED 1 /* FP32 LU on MFMA + FP64 iterative refinement (<= 3 sweeps on
+                                  net14, 1 on net86/net87); a particle with an iteration that does
+                                  not converge is integrated by the FP64 path instead */
 
 /* per-particle status written to nnet_batch.status */
 #define NNET_STATUS_OK 0      /* integrated to hydro_dt */
