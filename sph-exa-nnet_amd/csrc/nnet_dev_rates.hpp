@@ -17,6 +17,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "nnet_layout.hpp"
+
 namespace nnet_amd
 {
 namespace phys
@@ -45,11 +47,6 @@ __device__ __forceinline__ int temperature_bin(double T)
     return k;
 }
 
-// Output slots of the six hand-fitted rate groups.
-struct SpecialIdx
-{
-    int r3a_e, r3a_l, r24, r1216, r32, rcag_e, rcag_l, roag_e, roag_l;
-};
 
 // Powers of t9 used by the rate expressions (t9 clamped at 10), net14.cpp:196-213.
 struct RatePow
