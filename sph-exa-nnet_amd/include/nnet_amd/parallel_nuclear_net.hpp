@@ -17,6 +17,13 @@
  * All particle arrays stay in device memory; the call is stream-ordered on the default stream
  * and synchronises before it returns, like the reference's GPU branch.
  *
+ * Functor objects are mapped to the device parameterization by two traits, RatesKind<T> and
+ * EosDesc<T> (below).  They are specialized here for this header's functor types;
+ * nnet_amd/reference_binding.hpp specializes them for the reference's own polymorphic objects
+ * (nnet::net14/86/87::ComputeReactionRatesFunctor<double>, nnet::eos::IdealGasFunctor<double>,
+ * nnet::eos::HelmholtzFunctor<double>, seen through their nnet:: base classes), and a caller can
+ * specialize them for functor types of its own.
+ *
  * jumpToNse is accepted for signature compatibility and ignored, as in the reference
  * (parallel_nuclear_net_impl.cpp:63-103 never calls it for the solve).
  */
@@ -25,6 +32,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstdlib>
+#include <iostream>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -151,6 +159,41 @@ private:
     int code_;
 };
 
+/* ---- functor traits ----
+ * RatesKind<T>::kind(f): the rate parameterization of functor f (NNET_RATES_NET14/86/87); throws
+ * Error(NNET_E_ARG) for a functor the device does not implement.
+ * EosDesc<T>::describe(e, b): fills b.eos and b.mu (ideal gas) or b.Z and b.z_dim (Helmholtz; Z
+ * is read during the call only). */
+template<class T, class Enable = void>
+struct RatesKind;
+template<int K>
+struct RatesKind<ComputeReactionRatesFunctor<K>>
+{
+    static int kind(const ComputeReactionRatesFunctor<K>&) { return K; }
+};
+
+template<class T, class Enable = void>
+struct EosDesc;
+template<>
+struct EosDesc<eos::IdealGasFunctor>
+{
+    static void describe(const eos::IdealGasFunctor& e, nnet_batch& b)
+    {
+        b.eos = NNET_EOS_IDEAL;
+        b.mu  = e.mu;
+    }
+};
+template<>
+struct EosDesc<eos::HelmholtzFunctor>
+{
+    static void describe(const eos::HelmholtzFunctor& e, nnet_batch& b)
+    {
+        b.eos   = NNET_EOS_HELMHOLTZ;
+        b.Z     = e.Z.data();
+        b.z_dim = (int)e.Z.size();
+    }
+};
+
 namespace detail
 {
 template<class P>
@@ -159,6 +202,23 @@ double* raw(P p)
     if constexpr (std::is_pointer_v<P>) return const_cast<double*>(reinterpret_cast<const double*>(p));
     else
         return reinterpret_cast<double*>(p.get());
+}
+
+/* Is n.devData.rho_m1 allocated over [0, lastIndex)?  Like the reference
+ * (parallel_nuclear_net.hpp:99-112): through fieldNames / isAllocated(index) when the Data type
+ * has them, else by the array's size. */
+template<class Dev>
+auto rho_m1_allocated(const Dev& dev, size_t lastIndex, int) -> decltype(dev.isAllocated(0), dev.fieldNames.begin(), bool())
+{
+    size_t idx = 0;
+    for (auto it = dev.fieldNames.begin(); it != dev.fieldNames.end() && std::string(*it) != "rho_m1"; ++it)
+        ++idx;
+    return dev.isAllocated(idx) && dev.rho_m1.size() >= lastIndex;
+}
+template<class Dev>
+bool rho_m1_allocated(const Dev& dev, size_t lastIndex, long)
+{
+    return dev.rho_m1.size() >= lastIndex && dev.rho_m1.size() > 0;
 }
 
 inline void check(nnet_ctx* ctx, int rc, const char* where)
@@ -286,27 +346,29 @@ struct Options
     uint32_t* fp64_solves = nullptr; // device (optional, mixed precision)
     const uint32_t* cost_hint = nullptr; // device (optional): expected cost per particle, e.g. newton of the last call
     const uint32_t* fp64_hint = nullptr; // device (optional, mixed precision): nonzero -> FP64 path, e.g. fp64_solves of the last call
-    // what use_drhodt = true means: NNET_DRHODT_CPU (the reference CPU path, rho_m1 overwritten by
-    // drho/dt, integration with drho/dt = 0) or NNET_DRHODT_INTEGRATE (the reference GPU branch:
-    // drho/dt drives rho(t), rho_m1 unchanged)
-    int drhodt_mode = NNET_DRHODT_CPU;
+    // what use_drhodt = true means: NNET_DRHODT_INTEGRATE (default: the reference's GPU branch,
+    // parallel_nuclear_net.cu:79-81, which this use_gpu interface replaces -- drho/dt drives
+    // rho(t), rho_m1 unchanged) or NNET_DRHODT_CPU (the reference CPU path,
+    // parallel_nuclear_net_impl.cpp:89-92 -- rho_m1 overwritten by drho/dt, integration with
+    // drho/dt = 0)
+    int drhodt_mode = NNET_DRHODT_INTEGRATE;
 };
 
-template<bool use_gpu, class Data, typename Float, class List, int KIND, class Eos, class nseFunction = void*>
+template<bool use_gpu, class Data, typename Float, class List, class Rates, class Eos, class nseFunction = void*>
 void computeNuclearReactions(Data& n, size_t firstIndex, size_t lastIndex, const Float hydro_dt,
-                             const Float previous_dt, const List& reactions,
-                             const ComputeReactionRatesFunctor<KIND>& construct_rates_BE, const Eos& eos,
-                             bool use_drhodt, const nseFunction jumpToNse = nullptr, const Options& opt = Options{})
+                             const Float previous_dt, const List& reactions, const Rates& construct_rates_BE,
+                             const Eos& eos, bool use_drhodt, const nseFunction jumpToNse = nullptr,
+                             const Options& opt = Options{})
 {
     static_assert(use_gpu, "this implementation runs on the GPU; the reference's CPU branch is not provided");
     static_assert(std::is_same_v<Float, double>, "the rate and EOS functors are FP64");
-    (void)construct_rates_BE;
     (void)jumpToNse;
     if (lastIndex <= firstIndex) return;
-    auto&                       reg = detail::Registry::get();
+    const int                   kind = RatesKind<Rates>::kind(construct_rates_BE);
+    auto&                       reg  = detail::Registry::get();
     std::lock_guard<std::mutex> lk(reg.mu);
-    nnet_ctx*                   c   = reg.context(opt.device);
-    nnet_network*               net = detail::network(c, opt.device, KIND, reactions);
+    nnet_ctx*                   c    = reg.context(opt.device);
+    nnet_network*               net  = detail::network(c, opt.device, kind, reactions);
 
     auto&               dev = n.devData;
     const int           dimension = (int)n.numSpecies;
@@ -319,6 +381,13 @@ void computeNuclearReactions(Data& n, size_t firstIndex, size_t lastIndex, const
     b.precision = opt.precision;
     b.n         = lastIndex - firstIndex;
     b.rho       = detail::raw(dev.rho.data()) + firstIndex;
+    // parallel_nuclear_net.hpp:99-116: drho/dt is switched off, with the reference's message,
+    // when rho_m1 is not allocated; the pointer is formed only for an allocated array
+    if (use_drhodt && !detail::rho_m1_allocated(dev, lastIndex, 0))
+    {
+        use_drhodt = false;
+        std::cerr << "disabeling using drho/dt because 'rho_m1' isn't alocated !\n";
+    }
     b.rho_m1    = use_drhodt ? detail::raw(dev.rho_m1.data()) + firstIndex : nullptr;
     b.Y         = Y.data();
     b.temp      = detail::raw(dev.temp.data()) + firstIndex;
@@ -329,19 +398,7 @@ void computeNuclearReactions(Data& n, size_t firstIndex, size_t lastIndex, const
     b.fp64_solves = opt.fp64_solves;
     b.cost_hint   = opt.cost_hint;
     b.fp64_hint   = opt.fp64_hint;
-    if constexpr (std::is_same_v<Eos, eos::IdealGasFunctor>)
-    {
-        b.eos = NNET_EOS_IDEAL;
-        b.mu  = eos.mu;
-    }
-    else
-    {
-        static_assert(std::is_same_v<Eos, eos::HelmholtzFunctor>, "EOS functor: IdealGasFunctor or HelmholtzFunctor");
-        b.eos   = NNET_EOS_HELMHOLTZ;
-        b.Z     = eos.Z.data();
-        b.z_dim = (int)eos.Z.size();
-    }
-    if (use_drhodt && b.rho_m1 == nullptr) use_drhodt = false; // the reference disables it with a warning
+    EosDesc<Eos>::describe(eos, b);
     detail::check(c, nnet_compute_sync(c, &b, 1, hydro_dt, previous_dt, use_drhodt ? opt.drhodt_mode : NNET_DRHODT_OFF),
                   "nnet_compute");
 }

@@ -18,12 +18,19 @@ cancellation and a last-ulp change of Y moves Ye = zbar/abar at O(1) -- so the G
 calibrated bar of assert_parity(r, r_pert): no worse than the 1-ulp-perturbed oracle on the same
 inputs.  The mixed-precision path is held to |dX| <= 1e-6 |X| + 1e-14, |dT| <= 1e-8 |T|.
 
-"Chaotic-controller" particles are set apart: the reference controller can bounce between two
-rejected timesteps from one accepted state (nuclear_net.hpp:744-757, 899-904) and leave that
-cycle only when rounding drift happens to produce an acceptable step -- after a number of
-attempts that changes completely with a last-ulp difference.  A particle whose oracle run
-contains more than CHAOTIC_RUN consecutive rejected attempts, or that reaches the Newton cap on
-either side, is counted and reported, but its status and counts are not required to match.
+"Chaotic-controller" particles are checked by their own rules: the reference controller can
+bounce between two rejected timesteps from one accepted state (nuclear_net.hpp:744-757, 899-904)
+and leave that cycle only when rounding drift happens to produce an acceptable step -- after a
+number of attempts that changes completely with a last-ulp difference.  A particle is chaotic
+when the oracle's run contains more than CHAOTIC_RUN consecutive rejected attempts or the oracle
+stops it at the Newton cap (a device-side cap where the oracle finishes is a status mismatch of a
+regular particle).  For chaotic particles the comparison counts
+  chaos_status_mismatch  a different status,
+  chaos_state_far        same status and accepted substeps, state outside tier 2,
+  chaos_bad              a non-finite state, T <= 0, or |sum X_gpu - sum X_oracle| > 1e-10,
+and the bar requires chaos_bad == 0 and the two counts no larger than the 1-ulp-perturbed
+oracle's (calibrated bar) or zero (absolute bar).  frac_northstar is the fraction of all particles,
+chaotic ones included, that meet tier 1.
 """
 import numpy as np
 
@@ -37,12 +44,28 @@ def compare(A, Y_gpu, T_gpu, st_gpu, sub_gpu, new_gpu, Y_orc, T_orc, st_orc, sub
             t_rtol=T_RTOL, max_run_orc=None):
     """A: mass numbers (d,).  Y_*: d x n.  Returns a dict of counts, fractions and max errors over
     the regular particles, plus the number of chaotic-controller particles."""
-    chaotic = (st_orc == 4) | (st_gpu == 4)
+    A = np.asarray(A)
+    chaotic = st_orc == 4
     if max_run_orc is not None:
         chaotic |= max_run_orc > CHAOTIC_RUN
     regular = ~chaotic
     n_chaotic = int(chaotic.sum())
-    A = np.asarray(A)
+    n_all = len(T_gpu)
+    # chaotic particles
+    Xg_all, Xo_all = Y_gpu * A[:, None], Y_orc * A[:, None]
+    sxg, sxo = Xg_all.sum(0), Xo_all.sum(0)
+    c_same = chaotic & (st_gpu == st_orc)
+    c_same_sub = c_same & (sub_gpu == sub_orc)
+    dx_all = np.abs(Xg_all - Xo_all)
+    dT_all = np.abs(T_gpu - T_orc)
+    near2 = np.all(dx_all <= X_RTOL2 * np.abs(Xo_all) + X_ATOL, axis=0) & (dT_all <= T_RTOL2 * np.abs(T_orc))
+    finite = np.all(np.isfinite(Y_gpu), axis=0) & np.isfinite(T_gpu) & (T_gpu > 0)
+    sumx_close = np.abs(sxg - sxo) <= SUMX_TOL * np.maximum(1.0, np.abs(sxo))
+    chaos_bad = chaotic & ~(finite & sumx_close)
+    # tier 1 over all particles (the north-star bar)
+    t1_all = ((st_gpu == st_orc) & (sub_gpu == sub_orc) & (new_gpu == new_orc)
+              & np.all(dx_all <= x_rtol * np.abs(Xo_all) + X_ATOL, axis=0) & (dT_all <= t_rtol * np.abs(T_orc))
+              & sumx_close)
     Y_gpu, Y_orc = Y_gpu[:, regular], Y_orc[:, regular]
     T_gpu, T_orc = T_gpu[regular], T_orc[regular]
     st_gpu, st_orc, sub_gpu, sub_orc = st_gpu[regular], st_orc[regular], sub_gpu[regular], sub_orc[regular]
@@ -64,7 +87,12 @@ def compare(A, Y_gpu, T_gpu, st_gpu, sub_gpu, new_gpu, Y_orc, T_orc, st_orc, sub
     mc = same_counts
     return dict(
         n=n,
+        n_all=n_all,
+        frac_northstar=float(t1_all.sum()) / max(n_all, 1),
         n_chaotic=n_chaotic,
+        chaos_status_mismatch=int((chaotic & ~c_same).sum()),
+        chaos_state_far=int((c_same_sub & ~near2).sum()),
+        chaos_bad=int(chaos_bad.sum()),
         n_ok=int((st_orc == 0).sum()),
         n_stalled=int((st_orc == 2).sum()),
         n_skipped=int((st_orc == 1).sum()),
@@ -85,13 +113,19 @@ def assert_parity(r, r_pert=None, tier1_min=TIER1_MIN_FRAC, counts_min=COUNTS_MI
     and a small count), and statuses must agree."""
     import math
     n = max(r["n"], 1)
+    assert r["chaos_bad"] == 0, r
     if r_pert is None:
         assert r["status_mismatch"] == 0, r
         assert r["count_mismatch"] <= (1 - counts_min) * n, r
         assert r["tier1"] >= tier1_min * n, r
         assert r["tier2_fail"] == 0, r
+        assert r["chaos_status_mismatch"] == 0, r
+        assert r["chaos_state_far"] == 0, r
         return
     slack = lambda frac: math.ceil(frac * n)
+    nc = r["n_chaotic"]
+    assert r["chaos_status_mismatch"] <= 2 * r_pert["chaos_status_mismatch"] + math.ceil(0.02 * nc), (r, r_pert)
+    assert r["chaos_state_far"] <= 2 * r_pert["chaos_state_far"] + math.ceil(0.02 * nc), (r, r_pert)
     assert r["status_mismatch"] <= 2 * r_pert["status_mismatch"] + slack(0.002), (r, r_pert)
     assert r["count_mismatch"] <= 2 * r_pert["count_mismatch"] + slack(0.005), (r, r_pert)
     assert n - r["tier1"] <= 2 * (r_pert["n"] - r_pert["tier1"]) + slack(0.01), (r, r_pert)

@@ -1,23 +1,42 @@
-"""A multi-step hydro loop over a mixed-network batch (net14 + net86 + net87 in one call, per-step
-dt log-uniform in [1e-6, 1e-3]) captured once in a HIP graph and replayed: every replay gives the
-results of the same loop run eagerly.  The library records no memset and no allocation while
-capturing (the work queue and the cost-ordering counters are cleared by kernels, and its device
-buffers only grow in calls that need more than before)."""
+"""The C5 shape: a multi-step hydro loop over a mixed-network batch (net14 + net86 + net87 in one
+call, per-step dt log-uniform in [1e-6, 1e-3], seeded) captured once in a HIP graph and replayed.
+Every replay gives the results of the same loop run eagerly, and those results match the oracle
+stepping each batch through the same dts (calibrated bar of tests/parity.py).  The library records
+no memset and no allocation while capturing (the work queue and the cost-ordering counters are
+cleared by kernels, and its device buffers only grow in calls that need more than before).
+
+The loop runs the FP64 path: a captured multi-batch call with a mixed-precision batch is refused
+with NNET_E_STATE (a host fault once seen in that combination has no shown cause yet), which the
+last test checks."""
 import os
 
 import numpy as np
 import pytest
 
 import workloads
+from parity import assert_parity, compare, summary
 
 pytestmark = pytest.mark.gpu
 
 TABLE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sph-exa-nnet_amd", "data",
                      "helm_table.bin")
 STEPS = 3
+NTHREADS = min(16, os.cpu_count() or 1)
 
 
-def test_captured_mixed_network_loop_replays_like_eager():
+def _oracle_loop(oracle_lib, c, dts, perturb):
+    oracle_lib.helm_load(TABLE)
+    oracle_lib.L.oracle_set_ulp_perturbation(1 if perturb else 0)
+    Y = np.ascontiguousarray(c["Y"].copy())
+    T, dt, rho = c["temp"].copy(), c["dt"].copy(), c["rho"].copy()
+    for h in dts:
+        st, sub, new = oracle_lib.batch(c["net"], rho, T, Y, dt, float(h), eos_kind=c["eos"], mu=c["mu"], Z=c["Z"],
+                                        nthreads=NTHREADS)
+    oracle_lib.L.oracle_set_ulp_perturbation(0)
+    return Y, T, st, sub, new, oracle_lib.last_max_run.copy()
+
+
+def test_captured_mixed_network_loop_replays_like_eager(oracle_lib):
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
@@ -67,4 +86,39 @@ def test_captured_mixed_network_loop_replays_like_eager():
             for k, v in r.items():
                 assert np.array_equal(st[k].cpu().numpy(), v, equal_nan=True), (rep, k)
         print(f"replay {rep} matches the eager loop", flush=True)
+    ctx.close()
+    # the eager (= replayed) results against the oracle stepping the same dts
+    for c, r in zip(parts, ref):
+        o = _oracle_loop(oracle_lib, c, dts, False)
+        p = _oracle_loop(oracle_lib, c, dts, True)
+        A = workloads.mass_numbers(c["net"])
+        res = compare(A, r["Y"], r["temp"], r["status"], r["substeps"].astype(np.uint32),
+                      r["newton"].astype(np.uint32), o[0], o[1], o[2], o[3], o[4], max_run_orc=o[5])
+        rp = compare(A, p[0], p[1], p[2], p[3], p[4], o[0], o[1], o[2], o[3], o[4], max_run_orc=o[5])
+        print(f"C5 {c['label']} batch after {STEPS} captured steps, gpu vs oracle:", summary(res), flush=True)
+        print(f"C5 {c['label']} batch, perturbed oracle vs oracle:", summary(rp), flush=True)
+        assert_parity(res, rp)
+
+
+def test_captured_multi_batch_mixed_is_refused():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import nnet_mi355x as nm
+    parts, dts = workloads.config5(n=64, seed=78, steps=1)
+    print("graph: capture of a mixed-precision multi-batch call is refused", flush=True)
+    ctx = nm.Context(0, helm_table=TABLE)
+    sts = [nm.device_state(torch, c["net"], c["rho"], c["temp"], c["Y"], c["dt"]) for c in parts]
+    bs = [ctx.make_batch(c["net"], st, eos=c["eos"], Z=c["Z"], precision=nm.PRECISION_MIXED)
+          for c, st in zip(parts, sts)]
+    before = [{k: v.clone() for k, v in st.items()} for st in sts]
+    stream = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    with pytest.raises(nm.NnetError, match="-5"):
+        with torch.cuda.graph(graph, stream=stream):
+            ctx.compute(bs, float(dts[0]), stream=stream.cuda_stream, sync=False)
+    torch.cuda.synchronize()
+    for st, st0 in zip(sts, before):
+        for k in st0:
+            assert torch.equal(st[k], st0[k]), k  # nothing ran
     ctx.close()

@@ -93,3 +93,35 @@ def test_shim_matches_cabi_and_oracle(tmp_path, oracle_lib, cid, n):
     rp = compare(workloads.mass_numbers(cfg["net"]), Yp, Tp, sp, subp, newp, Y, T, so, subo, newo,
                  max_run_orc=oracle_lib.last_max_run)
     assert_parity(r, rp)
+
+
+REF_DRIVER = os.path.join(ROOT, "tests", "cpp", "ref_binding_driver")
+
+
+@pytest.mark.parametrize("cid,n", [(2, 256), (4, 64)])
+def test_reference_objects_match_cabi(tmp_path, cid, n):
+    """The reference's own objects (nnet::ReactionList, nnet::netXX::computeReactionRates,
+    nnet::eos::HelmholtzFunctor<double>) through nnet_amd/reference_binding.hpp give the C-ABI
+    path's results bit for bit.  The driver is built where the reference tree was present."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    if not os.path.exists(REF_DRIVER):
+        pytest.skip("tests/cpp/ref_binding_driver not built (needs the reference tree at build time)")
+    import nnet_mi355x as nm
+    cfg = workloads.config(cid, n=n, seed=991 + cid)
+    d = cfg["Y"].shape[0]
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_input(fin, cfg)
+    print(f"ref_binding_driver net{cfg['net']} n={n}", flush=True)
+    subprocess.run([REF_DRIVER, "run", str(cfg["net"]), str(cfg["eos"]), fin, fout, TABLE], check=True, timeout=120)
+    s, _ = _read_output(fout, n, d)
+    ctx = nm.Context(0, helm_table=TABLE)
+    st = nm.device_state(torch, cfg["net"], cfg["rho"], cfg["temp"], cfg["Y"], cfg["dt"])
+    ctx.compute([ctx.make_batch(cfg["net"], st, eos=cfg["eos"], mu=cfg["mu"], Z=cfg["Z"])], cfg["hydro_dt"])
+    torch.cuda.synchronize()
+    g = (st["Y"].cpu().numpy(), st["temp"].cpu().numpy(), st["dt"].cpu().numpy(), st["status"].cpu().numpy(),
+         st["substeps"].cpu().numpy().astype(np.uint32), st["newton"].cpu().numpy().astype(np.uint32))
+    ctx.close()
+    for a, c in zip(s, g):
+        assert np.array_equal(np.asarray(a), np.asarray(c), equal_nan=True)
