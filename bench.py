@@ -57,23 +57,70 @@ def device_config(torch, cid, n):
     return device_config_from(torch, workloads.config(cid, n=n, seed=workloads.SEED0 + cid))
 
 
-def cpu_baseline(cid, sample, steps_hint):
-    """The CPU restatement of the reference OpenMP path (oracle) on a bounded sample."""
+def host_cpus():
+    """(threads to use, physical cores available, model name): the physical cores of this process's
+    CPU affinity set (lscpu core ids), at most 16 (the host share of one GPU)."""
+    import subprocess
+    aff = sorted(os.sched_getaffinity(0))
+    cores, model = set(), "unknown"
+    try:
+        for line in subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True,
+                                   check=True).stdout.splitlines():
+            if line.startswith("#"):
+                continue
+            cpu, core, sock = (int(x) for x in line.split(",")[:3])
+            if cpu in aff:
+                cores.add((sock, core))
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        cores = set(aff)
+    phys = max(1, len(cores))
+    return min(16, phys), phys, model
+
+
+def stratified_sample(cfg, n, m):
+    """m particle indices of an n-particle configuration, taken evenly within each stratum (hot /
+    cool for C3, all particles otherwise) in proportion to the stratum's size."""
+    strata = [np.nonzero(cfg["hot"])[0], np.nonzero(~cfg["hot"])[0]] if "hot" in cfg else [np.arange(n)]
+    idx = []
+    for s in strata:
+        k = int(round(m * len(s) / n))
+        if k:
+            idx.append(s[np.linspace(0, len(s) - 1, k).astype(np.int64)])
+    return np.sort(np.concatenate(idx))
+
+
+def cpu_baseline(cfg, snap):
+    """The reference CPU path (its compiled substep solver in its OpenMP particle loop,
+    parallel_nuclear_net_impl.cpp:63-103, with the replay detection and Newton cap that let it
+    return on every particle; oracle/_ref) -- or, where that build is absent, the oracle
+    restatement -- on `snap`: a stratified subsample of the device state taken at the start of the
+    first timed step, integrated over one hydro step on this host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle
-    o = Oracle()
-    o.helm_load(TABLE)
-    cfg = workloads.config(cid, n=sample, seed=workloads.SEED0 + 100 + cid)
-    Y = np.ascontiguousarray(cfg["Y"])
-    T, dt, rho = cfg["temp"].copy(), cfg["dt"].copy(), cfg["rho"].copy()
-    threads = os.cpu_count() or 1
-    t0 = time.perf_counter()
-    st, sub, new = o.batch(cfg["net"], rho, T, Y, dt, cfg["hydro_dt"], eos_kind=cfg["eos"], mu=cfg["mu"],
-                           Z=cfg["Z"], nthreads=threads)
+    import oracle as orc
+    threads, phys, model = host_cpus()
+    Y = np.ascontiguousarray(snap["Y"])
+    T, dt, rho = snap["temp"].copy(), snap["dt"].copy(), snap["rho"].copy()
+    m = len(rho)
+    if orc.reference_available() and orc.Reference().has_helmholtz:
+        lib, kind = orc.Reference(), "reference"
+        t0 = time.perf_counter()
+        st, new = lib.batch_guarded(cfg["net"], rho, T, Y, dt, cfg["hydro_dt"], eos_kind=cfg["eos"], mu=cfg["mu"],
+                                    Z=cfg["Z"], nthreads=threads, max_iter=snap["max_iter"])
+    else:
+        lib, kind = orc.Oracle(), "port"
+        lib.helm_load(TABLE)
+        t0 = time.perf_counter()
+        st, _, new = lib.batch(cfg["net"], rho, T, Y, dt, cfg["hydro_dt"], eos_kind=cfg["eos"], mu=cfg["mu"],
+                               Z=cfg["Z"], nthreads=threads, max_iter=snap["max_iter"])
     el = time.perf_counter() - t0
-    return {"value": sample / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"first hydro step of {sample} particles of the same distribution (seed +100), "
-                      f"{el:.2f} s, statuses {np.bincount(st, minlength=5).tolist()}"}
+    return {"value": m / el, "unit": "particle-steps/s", "cores": threads, "kind": kind,
+            "sample": f"{m} particles, stratified, of the device state at the start of the first timed FP64 step, "
+                      f"one hydro step of {cfg['hydro_dt']} s, {el:.2f} s on {threads} threads "
+                      f"({phys} physical cores available, {model}); statuses "
+                      f"{np.bincount(st, minlength=5).tolist()}, Newton iterations {int(new.sum())}"}
 
 
 def run_c5(args, torch, nm):
@@ -81,8 +128,14 @@ def run_c5(args, torch, nm):
     hydro step; `--steps` steps with per-step dt log-uniform in [1e-6, 1e-3] are captured once in
     a HIP graph; the timed region is `--warmup` untimed replays then one timed replay of the loop
     (a replay continues from the state the previous one left)."""
+    if args.precision != "fp64":
+        # nnet_compute refuses a mixed-precision batch in a captured multi-batch call: a host fault
+        # once seen in that combination has no shown cause yet
+        print("bench.py --config 5 runs the FP64 path (--precision fp64): mixed precision in a captured "
+              "multi-batch call is disabled", file=sys.stderr)
+        sys.exit(2)
     parts, dts = workloads.config5(n=args.n, steps=args.steps)
-    prec = nm.PRECISION_FP64 if args.precision == "fp64" else nm.PRECISION_MIXED
+    prec = nm.PRECISION_FP64
     ctx = nm.Context(0, helm_table=TABLE)
     sts = []
     for c in parts:
@@ -242,8 +295,12 @@ def main():
     ap.add_argument("--workload", default="solve", choices=["solve", "helm-eos"],
                     help="solve (default): computeNuclearReactions; helm-eos: the post-step EOS refresh "
                          "computeHelmEOS over the same particle set")
-    ap.add_argument("--cpu-sample", type=int, default=524288,
-                    help="particles of the CPU-baseline sample (the oracle on all host cores, ~10-30 s)")
+    ap.add_argument("--cpu-sample", type=int, default=65536,
+                    help="particles of the CPU-baseline sample: a stratified subsample of the device state at the "
+                         "start of the first timed step, one hydro step on the host's physical cores (~10-30 s)")
+    ap.add_argument("--max-iter", type=int, default=10000,
+                    help="Newton-iteration cap per particle and step (nnet_ctx_set_max_iterations; the oracle and "
+                         "the CPU baseline use the same cap)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times): the reference kernel whose "
@@ -277,10 +334,14 @@ def main():
         return
     n = args.n or {2: 1048576, 3: 2097152, 4: 2097152}[args.config]
     ctx = nm.Context(0, helm_table=TABLE)
+    ctx.set_max_iterations(args.max_iter)
     stream = torch.cuda.current_stream()
+    slots = ctx.resident_slots({2: 14, 3: 86, 4: 87}[args.config])
 
-    def run(precision):
-        """Warmup + timed hydro steps of one precision from the seeded initial state."""
+    def run(precision, snapshot):
+        """Warmup + timed hydro steps of one precision from the seeded initial state.  Returns the
+        measurements and the final device state; with snapshot=True also a stratified subsample of
+        the state at the start of the first timed step (the CPU baseline's input)."""
         cfg, st = device_config(torch, args.config, n)
         prec = nm.PRECISION_FP64 if precision == "fp64" else nm.PRECISION_MIXED
         # each step dispatches the particles in descending order of the previous step's Newton
@@ -291,14 +352,19 @@ def main():
         st["fp64_hint"] = st["fp64_solves"]
         batch = ctx.make_batch(cfg["net"], st, eos=cfg["eos"], mu=cfg["mu"], Z=cfg["Z"], precision=prec)
         hdt = cfg["hydro_dt"]
-        print(f"# {cfg['label']}: n={n} precision={precision} warmup={args.warmup} steps={args.steps}",
-              file=sys.stderr, flush=True)
+        print(f"# {cfg['label']}: n={n} precision={precision} warmup={args.warmup} steps={args.steps} "
+              f"cap={args.max_iter}", file=sys.stderr, flush=True)
         for w in range(args.warmup):
             ctx.compute([batch], hdt, stream=stream.cuda_stream, sync=False)
             torch.cuda.synchronize()
             print(f"# {precision} warmup step {w} done", file=sys.stderr, flush=True)
+        snap = None
+        if snapshot:
+            idx = torch.from_numpy(stratified_sample(cfg, n, min(args.cpu_sample, n))).cuda()
+            snap = {k: st[k][..., idx].cpu().numpy() for k in ("rho", "temp", "dt", "Y")}
+            snap["max_iter"] = args.max_iter
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-        newton_total = fp64_total = 0
+        newton_total = fp64_total = capped_newton = 0
         status_hist = np.zeros(5, np.int64)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -306,34 +372,74 @@ def main():
             ev[k][0].record(stream)
             ctx.compute([batch], hdt, stream=stream.cuda_stream, sync=False)
             ev[k][1].record(stream)
-            # counters of this step (tiny D2H reads after the kernel on the same stream)
+            # counters of this step (small device reductions and one status copy after the step)
             newton_total += int(st["newton"].sum().item())
             fp64_total += int(st["fp64_solves"].sum().item())
+            capped_newton += int(st["newton"][st["status"] == 4].sum().item())
             status_hist += np.bincount(st["status"].cpu().numpy(), minlength=5)
             print(f"# {precision} step {k} done", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         kernel_s = float(np.mean([x.elapsed_time(y) for x, y in ev])) / 1e3
-        flops_per_launch = newton_total / args.steps * flops_per_iter(cfg["net"])
-        return cfg, {"particle_steps_per_s": n * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
-                     "kernel_ms": kernel_s * 1e3, "achieved_tflops": flops_per_launch / kernel_s / 1e12,
-                     "newton_iters_per_step": newton_total / args.steps,
-                     "fp64_resolves_per_step": fp64_total / args.steps,
-                     "status_per_step": (status_hist / args.steps).tolist()}
+        it_step = newton_total / args.steps
+        flops_per_launch = it_step * flops_per_iter(cfg["net"])
+        out = {"particle_steps_per_s": n * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
+               "kernel_ms": kernel_s * 1e3, "achieved_tflops": flops_per_launch / kernel_s / 1e12,
+               "newton_iters_per_step": it_step, "newton_iters_per_s": it_step / kernel_s,
+               "fp64_resolves_per_step": fp64_total / args.steps,
+               "status_per_step": (status_hist / args.steps).tolist(),
+               "max_iter_particles_per_step": float(status_hist[4]) / args.steps,
+               "max_iter_share_of_iterations": capped_newton / max(newton_total, 1),
+               # resident workgroups x kernel time / iterations: the mean latency of one Newton
+               # iteration of one particle (the work queue keeps every slot busy until the tail)
+               "iteration_latency_us": slots * kernel_s / max(it_step, 1) * 1e6}
+        final = {k: st[k] for k in ("Y", "temp", "status", "newton")}
+        return cfg, out, final, snap
+
+    def precision_errors(ref, mix, cfg):
+        """Per-output difference of the mixed path's final state from the FP64 path's (both from the
+        same seeded state after the same steps), over the particles with the same status that
+        neither path stopped at the Newton cap: mass fractions X = Y A (relative, floor 1e-14),
+        temperature (relative) and specific energy release eps = sum_i BE_i (Y_i - Y0_i)."""
+        A = torch.tensor(workloads.mass_numbers(cfg["net"]), dtype=torch.float64, device="cuda:0")[:, None]
+        BE = torch.tensor(workloads.binding_energies(cfg["net"]), dtype=torch.float64, device="cuda:0")
+        Y0 = torch.tensor(cfg["Y"], dtype=torch.float64, device="cuda:0")
+        sel = (ref["status"] == mix["status"]) & (ref["status"] != 4) & (ref["status"] != 1)
+        Xr, Xm = ref["Y"][:, sel] * A, mix["Y"][:, sel] * A
+        relx = ((Xm - Xr).abs() / Xr.abs().clamp_min(1e-14)).amax(0)
+        okx = ((Xm - Xr).abs() <= 1e-6 * Xr.abs() + 1e-14).all(0)
+        relT = (mix["temp"][sel] - ref["temp"][sel]).abs() / ref["temp"][sel].abs()
+        er = BE @ (ref["Y"][:, sel] - Y0[:, sel])
+        em = BE @ (mix["Y"][:, sel] - Y0[:, sel])
+        nz = er.abs() > 0
+        rele = (em - er)[nz].abs() / er[nz].abs()
+        m = int(sel.sum().item())
+        q = lambda t, f: float(t.max().item()) if f == "max" else float(t.median().item())
+        return {"particles_compared": m, "status_mismatch": int((ref["status"] != mix["status"]).sum().item()),
+                "X_max_rel": q(relx, "max") if m else 0.0, "X_median_rel": q(relx, "median") if m else 0.0,
+                "T_max_rel": q(relT, "max") if m else 0.0, "T_median_rel": q(relT, "median") if m else 0.0,
+                "eps_max_rel": q(rele, "max") if rele.numel() else 0.0,
+                "eps_median_rel": q(rele, "median") if rele.numel() else 0.0,
+                "within_1e-6_X_1e-8_T": float((okx & (relT <= 1e-8)).float().mean().item()) if m else 1.0}
 
     precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
-    res = {}
+    res, finals, snap = {}, {}, None
     for p in precisions:
-        cfg, res[p] = run(p)
-    head = res["mixed"] if "mixed" in res else res["fp64"]
-    achieved = head["achieved_tflops"] * 1e12
+        cfg, res[p], finals[p], sn = run(p, snapshot=(p == precisions[0] and not args.no_cpu))
+        snap = snap or sn
     head_prec = "mixed" if "mixed" in res else "fp64"
-    tb = TRAFFIC_BYTES_PER_PARTICLE_STEP.get((args.config, head_prec))
-    traffic = tb * n if tb is not None else None
-    conf = {"workload": cfg["label"], "particles": n, "hydro_dt": cfg["hydro_dt"], "precision": args.precision}
+    head = res[head_prec]
+    conf = {"workload": cfg["label"], "particles": n, "hydro_dt": cfg["hydro_dt"], "precision": args.precision,
+            "max_iter": args.max_iter, "resident_slots": slots}
     conf.update({p: r for p, r in res.items()})
     if "mixed" in res and "fp64" in res:
         conf["mixed_over_fp64"] = res["mixed"]["particle_steps_per_s"] / res["fp64"]["particle_steps_per_s"]
+        conf["mixed_vs_fp64_error"] = precision_errors(finals["fp64"], finals["mixed"], cfg)
+    # roofline of the dominant kernel, the FP64 solve (all of the FP64 path's time, and the
+    # routed and redo passes that carry most of the mixed path's)
+    dom = res["fp64"] if "fp64" in res else head
+    achieved = dom["achieved_tflops"] * 1e12
+    tb = TRAFFIC_BYTES_PER_PARTICLE_STEP.get((args.config, "fp64" if "fp64" in res else head_prec))
     out = {
         "metric": "net86 particle-steps/s, mixed vs FP64 on 1 GPU at <=1e-6 rel mass-frac error",
         "value": head["particle_steps_per_s"],
@@ -349,12 +455,15 @@ def main():
         "data": "synthetic",
         "config": conf,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": traffic},
+                     "frac": achieved / PEAK_FP64, "traffic": tb * n if tb is not None else None,
+                     "traffic_source": "constant: calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE of the profile run",
+                     "kernel": f"nnet_solve_kernel<{cfg['net']}, HELMHOLTZ, FP64>",
+                     # the kernel is latency x slot bound: iterations/s = slots / iteration latency
+                     "limiter": "latency",
+                     "latency_model": {"resident_slots": slots, "iteration_latency_us": dom["iteration_latency_us"],
+                                       "newton_iters_per_s": dom["newton_iters_per_s"]}},
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_sample, args.steps)
-    else:
-        out["cpu_baseline"] = None
+    out["cpu_baseline"] = cpu_baseline(cfg, snap) if snap is not None else None
     print(json.dumps(out), flush=True)
     ctx.close()
 

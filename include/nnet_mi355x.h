@@ -157,6 +157,12 @@ int nnet_compute(nnet_ctx* ctx, const nnet_batch* batches, int nb, double hydro_
  * accepted substep with status NNET_STATUS_MAX_ITER. */
 int nnet_ctx_set_max_iterations(nnet_ctx* ctx, long max_iter);
 
+/* Particles a solve launch integrates concurrently on this context's device: the workgroups of
+ * the persistent kernel for (rates_kind, eos, precision) that stay resident (occupancy per CU x
+ * CUs), one particle each.  No counterpart in the reference (whose launch is one thread per
+ * particle, parallel_nuclear_net.cu:144-156); for tools that model or tune the launch. */
+int nnet_solve_slots(nnet_ctx* ctx, int rates_kind, int eos, int precision, int* slots);
+
 /* Blocking variant of nnet_compute (drop-in for the reference's synchronous launch). */
 int nnet_compute_sync(nnet_ctx* ctx, const nnet_batch* batches, int nb, double hydro_dt,
                       double previous_dt, int use_drhodt);

@@ -169,6 +169,26 @@ class Reference(_Lib):
     def __init__(self, path=REF_SO):
         super().__init__(path, "ref_")
         self.L.ref_has_helmholtz.restype = C.c_int
+        self.L.ref_batch_guarded.argtypes = [C.c_int, C.c_int, C.c_double, D, C.c_int, C.c_size_t, D, C.POINTER(D), D,
+                                             D, C.c_double, C.c_long, C.c_int, C.POINTER(C.c_uint8),
+                                             C.POINTER(C.c_uint32)]
+
+    def batch_guarded(self, net, rho, temp, Y, dt, hydro_dt, eos_kind=0, mu=10.0, Z=None, nthreads=0,
+                      max_iter=10000):
+        """The reference's compiled substep solver in its OpenMP particle loop, with the oracle's
+        replay detection and Newton cap; in place on numpy arrays (Y: d x n, C-contiguous rows).
+        Returns (status, newton)."""
+        n = len(rho)
+        d = DIMS[net]
+        assert Y.shape == (d, n) and Y.flags["C_CONTIGUOUS"] and Y.dtype == np.float64
+        rows = (D * d)(*[Y[s].ctypes.data_as(D) for s in range(d)])
+        status = np.zeros(n, np.uint8)
+        newt = np.zeros(n, np.uint32)
+        Zc = np.ascontiguousarray(Z if Z is not None else np.zeros(1), dtype=np.float64)
+        self.L.ref_batch_guarded(net, eos_kind, mu, _p(Zc), len(Zc) if Z is not None else 0, n, _p(rho), rows,
+                                 _p(temp), _p(dt), hydro_dt, max_iter, nthreads,
+                                 status.ctypes.data_as(C.POINTER(C.c_uint8)), newt.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return status, newt
 
     @property
     def has_helmholtz(self):

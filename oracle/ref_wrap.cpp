@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "nnet/nuclear_net.hpp"
+#include "nnet_util/algorithm.hpp"
 #include "nnet/parameterization/eos/ideal_gas.hpp"
 #include "nnet/parameterization/net14/net14.hpp"
 #include "nnet/parameterization/net86/net86.hpp"
@@ -58,6 +59,70 @@ struct EosHolder
 };
 
 bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
+
+// The reference's substep loop (nuclear_net.hpp:931-959) with its own prepareSystemSubstep,
+// eigen::solve and finalizeSystemSubstep, plus the replay detection and Newton cap of the oracle
+// (the reference itself would not return from a replaying particle).  Caller-owned scratch.
+long guarded_particle(int net, const nnet::EosFunctor<double>& eos, double* Y, double* Tp, double rho, double drho_dt,
+                      double dt_tot, double* dtp, long max_iter, double* stats, double* trace, int trace_cap,
+                      double* rates, double* Mp, double* RHS, double* DY, double* Ybuf)
+{
+    const int             d = dim_for(net);
+    nnet::PtrReactionList L(list_for(net));
+    double&               T  = *Tp;
+    double&               dt = *dtp;
+    stats[0] = 0, stats[1] = 0, stats[2] = 0, stats[3] = 0;
+    long len = 0;
+    if (rho < nnet::constants::minRho || T < nnet::constants::minTemp) return 0;
+    double elapsed = 0, T_buf = std::nan("");
+    double att_dt = 0, att_T = 0;
+    long   total = 0;
+    for (int i = 1;; ++i)
+    {
+        const int it = i;
+        if (i == 1) att_dt = dt, att_T = T_buf;
+        double used = dt;
+        if (dt_tot - elapsed < dt) used = dt_tot - elapsed;
+        const double el0 = elapsed;
+        try
+        {
+            nnet::prepareSystemSubstep(d, Mp, RHS, rates, L, rates_for(net), eos, Y, T, Ybuf, T_buf, rho, drho_dt,
+                                       dt_tot, elapsed, dt, i);
+        }
+        catch (...)
+        {
+            stats[0] = 3;
+            return len;
+        }
+        eigen::solve(Mp, RHS, DY, d + 1, (double)nnet::constants::epsilonSystem);
+        stats[2] += 1;
+        ++total;
+        const bool done = nnet::finalizeSystemSubstep(d, Y, T, Ybuf, T_buf, DY, dt_tot, elapsed, dt, i);
+        const bool conv = (i == 0) && (elapsed != el0 || done);
+        if (len < trace_cap && trace)
+        {
+            double* p = trace + 6 * len;
+            p[0] = it, p[1] = used, p[2] = el0, p[3] = dt, p[4] = T_buf, p[5] = conv ? 1 : 0;
+        }
+        ++len;
+        if (conv) stats[1] += 1;
+        if (done) return len;
+        if (i == 0 && !conv)
+        {
+            stats[3] += 1;
+            if (same_bits(dt, att_dt) && same_bits(T_buf, att_T))
+            {
+                stats[0] = 2;
+                return len;
+            }
+        }
+        if (max_iter > 0 && total >= max_iter)
+        {
+            stats[0] = 4;
+            return len;
+        }
+    }
+}
 } // namespace
 
 extern "C"
@@ -135,62 +200,53 @@ extern "C"
                       double drho_dt, double dt_tot, double* dtp, long max_iter, double* stats, double* trace,
                       int trace_cap)
     {
-        const int             d = dim_for(net);
-        EosHolder             h(mu, Z, zdim);
-        nnet::PtrReactionList L(list_for(net));
-        std::vector<double>   rates(list_for(net).size()), Mp((d + 1) * (d + 1)), RHS(d + 1), DY(d + 1), Ybuf(d);
-        double&               T  = *Tp;
-        double&               dt = *dtp;
-        stats[0] = 0, stats[1] = 0, stats[2] = 0, stats[3] = 0;
-        long len = 0;
-        if (rho < nnet::constants::minRho || T < nnet::constants::minTemp) return 0;
-        double elapsed = 0, T_buf = std::nan("");
-        double att_dt = 0, att_T = 0;
-        long   total = 0;
-        for (int i = 1;; ++i)
+        const int           d = dim_for(net);
+        EosHolder           h(mu, Z, zdim);
+        std::vector<double> rates(list_for(net).size()), Mp((d + 1) * (d + 1)), RHS(d + 1), DY(d + 1), Ybuf(d);
+        return guarded_particle(net, h.get(eos_kind), Y, Tp, rho, drho_dt, dt_tot, dtp, max_iter, stats, trace,
+                                trace_cap, rates.data(), Mp.data(), RHS.data(), DY.data(), Ybuf.data());
+    }
+
+    // The reference's OpenMP particle loop (parallel_nuclear_net_impl.cpp:63-103: skip test,
+    // schedule(dynamic, util::dynamicBatchSize), per-thread buffers) over guarded_particle, i.e.
+    // the reference's compiled substep solver with the replay detection and Newton cap that let it
+    // return on every input.  status: 0 ok, 1 skipped, 2 stalled, 3 zero dt, 4 capped.
+    void ref_batch_guarded(int net, int eos_kind, double mu, const double* Z, int zdim, size_t n, const double* rho,
+                           double** Y, double* temp, double* dt, double hydro_dt, long max_iter, int nthreads,
+                           uint8_t* status, uint32_t* newton)
+    {
+        const int d  = dim_for(net);
+        const int nR = (int)list_for(net).size();
+        EosHolder h(mu, Z, zdim);
+        if (nthreads > 0) omp_set_num_threads(nthreads);
+        int num_threads = 1;
+#pragma omp parallel
+#pragma omp master
+        num_threads = omp_get_num_threads();
+        const int chunk = util::dynamicBatchSize(n, num_threads);
+#pragma omp parallel
         {
-            const int it = i;
-            if (i == 1) att_dt = dt, att_T = T_buf;
-            double used = dt;
-            if (dt_tot - elapsed < dt) used = dt_tot - elapsed;
-            const double el0 = elapsed;
-            try
+            std::vector<double> rates(nR), Mp((d + 1) * (d + 1)), RHS(d + 1), DY(d + 1), Ybuf(d), Yl(d);
+            double              stats[4];
+#pragma omp for schedule(dynamic, chunk)
+            for (size_t i = 0; i < n; ++i)
             {
-                nnet::prepareSystemSubstep(d, Mp.data(), RHS.data(), rates.data(), L, rates_for(net), h.get(eos_kind), Y,
-                                           T, Ybuf.data(), T_buf, rho, drho_dt, dt_tot, elapsed, dt, i);
-            }
-            catch (...)
-            {
-                stats[0] = 3;
-                return len;
-            }
-            eigen::solve(Mp.data(), RHS.data(), DY.data(), d + 1, (double)nnet::constants::epsilonSystem);
-            stats[2] += 1;
-            ++total;
-            const bool done =
-                nnet::finalizeSystemSubstep(d, Y, T, Ybuf.data(), T_buf, DY.data(), dt_tot, elapsed, dt, i);
-            const bool conv = (i == 0) && (elapsed != el0 || done);
-            if (len < trace_cap && trace)
-            {
-                double* p = trace + 6 * len;
-                p[0] = it, p[1] = used, p[2] = el0, p[3] = dt, p[4] = T_buf, p[5] = conv ? 1 : 0;
-            }
-            ++len;
-            if (conv) stats[1] += 1;
-            if (done) return len;
-            if (i == 0 && !conv)
-            {
-                stats[3] += 1;
-                if (same_bits(dt, att_dt) && same_bits(T_buf, att_T))
+                uint8_t  st = 1;
+                uint32_t nw = 0;
+                if (rho[i] > nnet::constants::minRho && temp[i] > nnet::constants::minTemp)
                 {
-                    stats[0] = 2;
-                    return len;
+                    for (int j = 0; j < d; ++j)
+                        Yl[j] = Y[j][i];
+                    guarded_particle(net, h.get(eos_kind), Yl.data(), &temp[i], rho[i], 0., hydro_dt, &dt[i],
+                                     max_iter, stats, nullptr, 0, rates.data(), Mp.data(), RHS.data(), DY.data(),
+                                     Ybuf.data());
+                    for (int j = 0; j < d; ++j)
+                        Y[j][i] = Yl[j];
+                    st = (uint8_t)stats[0];
+                    nw = (uint32_t)stats[2];
                 }
-            }
-            if (max_iter > 0 && total >= max_iter)
-            {
-                stats[0] = 4;
-                return len;
+                if (status) status[i] = st;
+                if (newton) newton[i] = nw;
             }
         }
     }

@@ -394,16 +394,34 @@ struct TileLU
             const int j = X::tcol(wc + WR * tj, lc);
             Bop[tj]     = (j >= p0 + 4 && j <= N) ? RT(j, lr) : R(0);
         }
+        // The tiles holding the next block's panel (its tile row and tile column) are updated
+        // first and published; the other tiles' MFMAs are issued after the publish, so they
+        // execute while the wave waits at the barrier and factors the next diagonal block (each
+        // tile still receives its updates in block order).
+        constexpr int Tn = (4 * (STEP + 1)) >> 4;
+        auto update = [&](bool panel_tiles) __attribute__((always_inline))
+        {
 #pragma unroll
-        for (int ti = 0; ti < TI; ++ti)
+            for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-            for (int tj = 0; tj < TI; ++tj)
-            {
-                const int I = wr + WR * ti, J = wc + WR * tj;
-                if (I >= Tp && J >= Tp && 16 * I + 15 >= p0 + 4 && 16 * J + 15 >= p0 + 4)
-                    Mt[ti][tj] = TileT<R>::mfma(Aop[ti], Bop[tj], Mt[ti][tj]);
-            }
+                for (int tj = 0; tj < TI; ++tj)
+                {
+                    const int I = wr + WR * ti, J = wc + WR * tj;
+                    if (I >= Tp && J >= Tp && 16 * I + 15 >= p0 + 4 && 16 * J + 15 >= p0 + 4 &&
+                        ((I == Tn || J == Tn) == panel_tiles))
+                        Mt[ti][tj] = TileT<R>::mfma(Aop[ti], Bop[tj], Mt[ti][tj]);
+                }
+        };
+#ifdef NNET_LU_ISSUE_IN_ORDER
+        update(true);
+        update(false);
         if constexpr (STEP + 1 < NS) publish<STEP + 1>(Mt, pan, wr, wc, lr, lc);
+#else
+        update(true);
+        if constexpr (STEP + 1 < NS) publish<STEP + 1>(Mt, pan, wr, wc, lr, lc);
+        __builtin_amdgcn_sched_barrier(0);
+        update(false);
+#endif
         __syncthreads();
         if constexpr (STEP + 1 < NS) step<STEP + 1>(Mt, pan, sU, sL, sRinv, wr, wc, lr, lc, tid);
     }
@@ -1627,6 +1645,224 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
     return NNET_OK;
 }
 
+// Device buffers of one nnet_compute call: per batch NCTR work-queue counters and BIN_STRIDE
+// ordering bins, the dispatch order of the cost-ordered batches and the redo lists of the mixed
+// batches (each batch takes its n entries in batch order).
+struct CallBufs
+{
+    unsigned int* queue;
+    unsigned int* bins;
+    uint32_t*     order;
+    uint32_t*     redo;
+};
+struct Needs
+{
+    size_t order = 0, redo = 0; // entries
+};
+static Needs call_needs(const nnet_batch* batches, int nb)
+{
+    Needs n;
+    for (int bi = 0; bi < nb; ++bi)
+    {
+        const nnet_batch& b = batches[bi];
+        if (b.cost_hint || (b.fp64_hint && b.precision == NNET_PRECISION_MIXED)) n.order += b.n;
+        if (b.precision == NNET_PRECISION_MIXED) n.redo += b.n;
+    }
+    return n;
+}
+
+// Argument checks of nnet_compute; nothing is allocated or enqueued.  n_live: non-empty batches.
+static int check_call(nnet_ctx* c, const nnet_batch* batches, int nb, double hydro_dt, int use_drhodt, int& n_live)
+{
+    if (!c || (nb > 0 && !batches) || nb < 0) return NNET_E_ARG;
+    if (!(hydro_dt > 0)) return fail(c, NNET_E_ARG, "hydro_dt must be positive");
+    if (use_drhodt != NNET_DRHODT_OFF && use_drhodt != NNET_DRHODT_CPU && use_drhodt != NNET_DRHODT_INTEGRATE)
+        return fail(c, NNET_E_ARG, "use_drhodt must be NNET_DRHODT_OFF, _CPU or _INTEGRATE");
+    n_live = 0;
+    for (int bi = 0; bi < nb; ++bi)
+    {
+        const nnet_batch& b = batches[bi];
+        if (!b.network || b.network->ctx != c) return fail(c, NNET_E_ARG, "batch without a network of this context");
+        if (b.n == 0) continue;
+        ++n_live;
+        if (!b.rho || !b.Y || !b.temp || !b.dt || (use_drhodt != NNET_DRHODT_OFF && !b.rho_m1))
+            return fail(c, NNET_E_ARG, "null particle array");
+        if (b.precision != NNET_PRECISION_FP64 && b.precision != NNET_PRECISION_MIXED)
+            return fail(c, NNET_E_ARG, "unknown precision");
+        for (int z = 0; z < b.network->d; ++z)
+            if (!b.Y[z]) return fail(c, NNET_E_ARG, "null abundance array");
+        if (b.eos == NNET_EOS_HELMHOLTZ)
+        {
+            if (!c->helm_ok) return fail(c, NNET_E_STATE, "no Helmholtz table loaded");
+            if (!b.Z || b.z_dim <= 0 || b.z_dim > b.network->d) return fail(c, NNET_E_ARG, "Helmholtz Z/z_dim invalid");
+        }
+        else if (b.eos != NNET_EOS_IDEAL)
+            return fail(c, NNET_E_ARG, "unknown EOS");
+    }
+    return NNET_OK;
+}
+
+// Enqueues the kernels of one nnet_compute call on `st` with the given buffers.
+static int enqueue_compute(nnet_ctx* c, const nnet_batch* batches, int nb, double hydro_dt, double previous_dt,
+                           int use_drhodt, hipStream_t st, int n_live, const CallBufs& bufs)
+{
+    StepParams sp{hydro_dt, previous_dt, use_drhodt, c->max_iter};
+    const bool fork = n_live > 1;
+    // the fp64 routing hint forks an auxiliary stream per batch; it is honoured in single-batch
+    // calls outside HIP graph capture and ignored otherwise (results stay valid without it)
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusActive;
+    if (hipStreamIsCapturing(st, &capture) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        capture = hipStreamCaptureStatusActive; // unknown: treat as captured
+    }
+    const bool allow_route = !fork && capture == hipStreamCaptureStatusNone;
+    // A captured multi-batch call with a mixed-precision batch (the C5 hydro loop) ended in a
+    // host segmentation fault whose cause has not been shown; until it is, that combination
+    // is refused rather than run (the FP64 multi-batch capture is tested and stays enabled)
+    if (fork && capture != hipStreamCaptureStatusNone)
+        for (int bi = 0; bi < nb; ++bi)
+            if (batches[bi].n > 0 && batches[bi].precision == NNET_PRECISION_MIXED)
+                return fail(c, NNET_E_STATE,
+                            "mixed precision in a multi-batch call under HIP graph capture is disabled");
+    if (fork)
+    {
+        HIP_TRY(c, hipEventRecord(c->fork_ev, st));
+        for (int k = 0; k < std::min(n_live, (int)nnet_ctx::NSIDE); ++k)
+            HIP_TRY(c, hipStreamWaitEvent(c->side[k], c->fork_ev, 0));
+    }
+    size_t order_off = 0, redo_off = 0;
+    int    live      = 0;
+    for (int bi = 0; bi < nb; ++bi)
+    {
+        const nnet_batch& b = batches[bi];
+        if (b.n == 0) continue;
+        const hipStream_t bst = fork ? c->side[live++ % nnet_ctx::NSIDE] : st;
+        const int         d   = b.network->d;
+        EosDev            ed{};
+        ed.kind = b.eos;
+        ed.mu   = b.mu;
+        if (b.eos == NNET_EOS_HELMHOLTZ)
+        {
+            for (int z = 0; z < b.z_dim; ++z)
+                ed.Z[z] = b.Z[z];
+            ed.zdim  = b.z_dim;
+            ed.grid  = c->d_grid;
+            ed.cells = c->d_cells;
+        }
+        BatchDev bd{};
+        bd.n = b.n, bd.rho = b.rho, bd.rho_m1 = b.rho_m1, bd.temp = b.temp, bd.dt = b.dt;
+        bd.status = b.status, bd.substeps = b.substeps, bd.newton = b.newton, bd.fp64_solves = b.fp64_solves;
+        for (int z = 0; z < d; ++z)
+            bd.Y[z] = b.Y[z];
+        const NetDev& nd  = b.network->dev;
+        unsigned int* ctr = bufs.queue + (size_t)nnet_ctx::NCTR * bi;
+        bd.queue          = ctr;
+        hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, bst, ctr, nnet_ctx::NCTR);
+        const bool mix = b.precision == NNET_PRECISION_MIXED;
+        if (mix)
+        {
+            bd.redo       = bufs.redo + redo_off;
+            bd.redo_count = ctr + 4;
+            redo_off += b.n;
+        }
+        // ordered dispatch: longest first by the cost hint; in a mixed batch, the particles with
+        // a nonzero fp64 hint form the prefix [0, nH) of the order, the rest [nH, n)
+        const uint32_t*     route = (mix && allow_route) ? b.fp64_hint : nullptr;
+        const unsigned int* range = nullptr;
+        bd.order                  = nullptr;
+        if (b.cost_hint || route)
+        {
+            unsigned int* bins  = bufs.bins + (size_t)BIN_STRIDE * bi;
+            uint32_t*     order = bufs.order + order_off;
+            order_off += b.n;
+            const unsigned blocks = (unsigned)std::min<size_t>((b.n + 255) / 256, 2048);
+            hipLaunchKernelGGL(nnet_queue_reset, dim3((BIN_STRIDE + 255) / 256), dim3(256), 0, bst, bins,
+                               BIN_STRIDE);
+            hipLaunchKernelGGL(nnet_order_count, dim3(blocks), dim3(256), 0, bst, b.cost_hint, route, b.n, bins);
+            hipLaunchKernelGGL(nnet_order_offsets, dim3(1), dim3(64), 0, bst, bins);
+            hipLaunchKernelGGL(nnet_order_scatter, dim3(blocks), dim3(256), 0, bst, b.cost_hint, route, b.n, bins,
+                               order);
+            bd.order = order;
+            range    = bins + 2 * NKEY; // {0, nH, n}
+        }
+        const int         kind = b.network->kind;
+        const int         ak   = fork ? (live - 1) % nnet_ctx::NSIDE : 0;
+        const hipStream_t ast  = c->aux[ak];
+        hipError_t        herr = hipSuccess;
+        auto              chk  = [&](hipError_t e)
+        {
+            if (e != hipSuccess && herr == hipSuccess) herr = e;
+        };
+        auto              launch = [&](auto kern, int nt, const BatchDev& bdv, const StepParams& spv, hipStream_t s)
+        {
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, 0) != hipSuccess || per_cu < 1)
+                per_cu = 1;
+            const size_t wgs = std::min<size_t>(b.n, (size_t)per_cu * c->n_cu);
+            hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(nt), 0, s, nd, ed, bdv, spv);
+        };
+        // mixed batch: the routed particles [0, nH) go to an FP64 pass on the aux stream while
+        // the mixed pass takes [nH, n); the particles the mixed pass gives up go to a last FP64
+        // pass over its redo list (rho_m1 already updated by the mixed pass)
+        BatchDev routed_bd  = bd;
+        routed_bd.queue     = ctr + 1;
+        routed_bd.qrange    = range;
+        routed_bd.fp64_pass = 1;
+        BatchDev mixed_bd   = bd;
+        mixed_bd.qrange     = route ? range + 1 : nullptr;
+        BatchDev redo_bd    = bd;
+        redo_bd.queue       = ctr + 2;
+        redo_bd.order       = bd.redo;
+        redo_bd.qrange      = ctr + 3; // {0, redo count}
+        redo_bd.fp64_pass   = 1;
+        StepParams redo_sp  = sp;
+        if (sp.use_drhodt == NNET_DRHODT_CPU) redo_sp.use_drhodt = NNET_DRHODT_OFF; // rho_m1 already updated
+        auto dispatch       = [&](auto netc, auto eosc)
+        {
+            constexpr int NV = decltype(netc)::value, EV = decltype(eosc)::value;
+            if (!mix)
+            {
+                launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, bd, sp, bst);
+                return;
+            }
+            if (route)
+            {
+                chk(hipEventRecord(c->aux_fork[ak], bst));
+                chk(hipStreamWaitEvent(ast, c->aux_fork[ak], 0));
+                launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, routed_bd, sp, ast);
+            }
+            launch(nnet_solve_kernel<NV, EV, true>, Cfg<NV>::NT, mixed_bd, sp, bst);
+            launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, redo_bd, redo_sp, bst);
+            if (route)
+            {
+                chk(hipEventRecord(c->aux_join[ak], ast));
+                chk(hipStreamWaitEvent(bst, c->aux_join[ak], 0));
+            }
+        };
+        auto dispatch_eos = [&](auto netc)
+        {
+            if (b.eos == NNET_EOS_IDEAL) dispatch(netc, std::integral_constant<int, NNET_EOS_IDEAL>{});
+            else
+                dispatch(netc, std::integral_constant<int, NNET_EOS_HELMHOLTZ>{});
+        };
+        if (kind == 14) dispatch_eos(std::integral_constant<int, 14>{});
+        else if (kind == 86)
+            dispatch_eos(std::integral_constant<int, 86>{});
+        else
+            dispatch_eos(std::integral_constant<int, 87>{});
+        HIP_TRY(c, herr);
+        HIP_TRY(c, hipGetLastError());
+    }
+    if (fork)
+        for (int k = 0; k < std::min(n_live, (int)nnet_ctx::NSIDE); ++k)
+        {
+            HIP_TRY(c, hipEventRecord(c->side_ev[k], c->side[k]));
+            HIP_TRY(c, hipStreamWaitEvent(st, c->side_ev[k], 0));
+        }
+    return NNET_OK;
+}
+
 extern "C"
 {
     int nnet_ctx_create(nnet_ctx** out, int device, const char* helm_table_path)
@@ -1698,6 +1934,40 @@ extern "C"
     {
         if (!c || m < 0) return NNET_E_ARG;
         c->max_iter = m;
+        return NNET_OK;
+    }
+
+    int nnet_solve_slots(nnet_ctx* c, int rates_kind, int eos, int precision, int* slots)
+    {
+        if (!c || !slots) return NNET_E_ARG;
+        if (!(rates_kind == 14 || rates_kind == 86 || rates_kind == 87) ||
+            !(eos == NNET_EOS_IDEAL || eos == NNET_EOS_HELMHOLTZ) ||
+            !(precision == NNET_PRECISION_FP64 || precision == NNET_PRECISION_MIXED))
+            return fail(c, NNET_E_ARG, "nnet_solve_slots: unknown network, EOS or precision");
+        HIP_TRY(c, hipSetDevice(c->device));
+        int        per_cu = 0;
+        hipError_t e      = hipSuccess;
+        auto       occ    = [&](auto kern, int nt) { e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, 0); };
+        auto       pick   = [&](auto netc, auto eosc)
+        {
+            constexpr int NV = decltype(netc)::value, EV = decltype(eosc)::value;
+            if (precision == NNET_PRECISION_MIXED) occ(nnet_solve_kernel<NV, EV, true>, Cfg<NV>::NT);
+            else
+                occ(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT);
+        };
+        auto by_eos = [&](auto netc)
+        {
+            if (eos == NNET_EOS_IDEAL) pick(netc, std::integral_constant<int, NNET_EOS_IDEAL>{});
+            else
+                pick(netc, std::integral_constant<int, NNET_EOS_HELMHOLTZ>{});
+        };
+        if (rates_kind == 14) by_eos(std::integral_constant<int, 14>{});
+        else if (rates_kind == 86)
+            by_eos(std::integral_constant<int, 86>{});
+        else
+            by_eos(std::integral_constant<int, 87>{});
+        HIP_TRY(c, e);
+        *slots = std::max(per_cu, 1) * c->n_cu;
         return NNET_OK;
     }
 
@@ -1889,13 +2159,12 @@ extern "C"
     int nnet_compute(nnet_ctx* c, const nnet_batch* batches, int nb, double hydro_dt, double previous_dt,
                      int use_drhodt, void* stream)
     {
-        if (!c || (nb > 0 && !batches) || nb < 0) return NNET_E_ARG;
-        if (!(hydro_dt > 0)) return fail(c, NNET_E_ARG, "hydro_dt must be positive");
-        if (use_drhodt != NNET_DRHODT_OFF && use_drhodt != NNET_DRHODT_CPU && use_drhodt != NNET_DRHODT_INTEGRATE)
-            return fail(c, NNET_E_ARG, "use_drhodt must be NNET_DRHODT_OFF, _CPU or _INTEGRATE");
+        int n_live = 0;
+        int rc     = check_call(c, batches, nb, hydro_dt, use_drhodt, n_live);
+        if (rc != NNET_OK) return rc;
         HIP_TRY(c, hipSetDevice(c->device));
-        hipStream_t st = (hipStream_t)stream;
-        StepParams  sp{hydro_dt, previous_dt, use_drhodt, c->max_iter};
+        hipStream_t st   = (hipStream_t)stream;
+        const Needs need = call_needs(batches, nb);
         if (nb > c->n_queue)
         {
             // grows only in calls with more batches than before (a capture-safe caller keeps nb fixed)
@@ -1906,221 +2175,35 @@ extern "C"
             c->n_queue = std::max(nb, 8);
         }
         // cost-ordered dispatch buffers: grow only when a call needs more than before
+        if (need.order > c->n_order || (need.order > 0 && nb > c->n_bins))
         {
-            size_t need = 0;
-            for (int bi = 0; bi < nb; ++bi)
-                if (batches[bi].cost_hint ||
-                    (batches[bi].fp64_hint && batches[bi].precision == NNET_PRECISION_MIXED))
-                    need += batches[bi].n;
-            if (need > c->n_order || (need > 0 && nb > c->n_bins))
+            HIP_TRY(c, hipStreamSynchronize(st));
+            if (need.order > c->n_order)
             {
-                HIP_TRY(c, hipStreamSynchronize(st));
-                if (need > c->n_order)
-                {
-                    if (c->d_order) HIP_TRY(c, hipFree(c->d_order));
-                    c->d_order = nullptr;
-                    HIP_TRY(c, hipMalloc(&c->d_order, sizeof(uint32_t) * need));
-                    c->n_order = need;
-                }
-                if (nb > c->n_bins)
-                {
-                    if (c->d_bins) HIP_TRY(c, hipFree(c->d_bins));
-                    c->d_bins = nullptr;
-                    HIP_TRY(c, hipMalloc(&c->d_bins, sizeof(unsigned int) * BIN_STRIDE * std::max(nb, 8)));
-                    c->n_bins = std::max(nb, 8);
-                }
+                if (c->d_order) HIP_TRY(c, hipFree(c->d_order));
+                c->d_order = nullptr;
+                HIP_TRY(c, hipMalloc(&c->d_order, sizeof(uint32_t) * need.order));
+                c->n_order = need.order;
+            }
+            if (nb > c->n_bins)
+            {
+                if (c->d_bins) HIP_TRY(c, hipFree(c->d_bins));
+                c->d_bins = nullptr;
+                HIP_TRY(c, hipMalloc(&c->d_bins, sizeof(unsigned int) * BIN_STRIDE * std::max(nb, 8)));
+                c->n_bins = std::max(nb, 8);
             }
         }
         // redo lists of the mixed batches: grow only when a call needs more than before
+        if (need.redo > c->n_redo)
         {
-            size_t need = 0;
-            for (int bi = 0; bi < nb; ++bi)
-                if (batches[bi].precision == NNET_PRECISION_MIXED) need += batches[bi].n;
-            if (need > c->n_redo)
-            {
-                HIP_TRY(c, hipStreamSynchronize(st));
-                if (c->d_redo) HIP_TRY(c, hipFree(c->d_redo));
-                c->d_redo = nullptr;
-                HIP_TRY(c, hipMalloc(&c->d_redo, sizeof(uint32_t) * need));
-                c->n_redo = need;
-            }
+            HIP_TRY(c, hipStreamSynchronize(st));
+            if (c->d_redo) HIP_TRY(c, hipFree(c->d_redo));
+            c->d_redo = nullptr;
+            HIP_TRY(c, hipMalloc(&c->d_redo, sizeof(uint32_t) * need.redo));
+            c->n_redo = need.redo;
         }
-        // validate every batch before anything is enqueued
-        int n_live = 0;
-        for (int bi = 0; bi < nb; ++bi)
-        {
-            const nnet_batch& b = batches[bi];
-            if (!b.network || b.network->ctx != c) return fail(c, NNET_E_ARG, "batch without a network of this context");
-            if (b.n == 0) continue;
-            ++n_live;
-            if (!b.rho || !b.Y || !b.temp || !b.dt || (use_drhodt != NNET_DRHODT_OFF && !b.rho_m1))
-                return fail(c, NNET_E_ARG, "null particle array");
-            if (b.precision != NNET_PRECISION_FP64 && b.precision != NNET_PRECISION_MIXED)
-                return fail(c, NNET_E_ARG, "unknown precision");
-            for (int z = 0; z < b.network->d; ++z)
-                if (!b.Y[z]) return fail(c, NNET_E_ARG, "null abundance array");
-            if (b.eos == NNET_EOS_HELMHOLTZ)
-            {
-                if (!c->helm_ok) return fail(c, NNET_E_STATE, "no Helmholtz table loaded");
-                if (!b.Z || b.z_dim <= 0 || b.z_dim > b.network->d) return fail(c, NNET_E_ARG, "Helmholtz Z/z_dim invalid");
-            }
-            else if (b.eos != NNET_EOS_IDEAL)
-                return fail(c, NNET_E_ARG, "unknown EOS");
-        }
-        const bool fork = n_live > 1;
-        // the fp64 routing hint forks an auxiliary stream per batch; it is honoured in single-batch
-        // calls outside HIP graph capture and ignored otherwise (results stay valid without it)
-        hipStreamCaptureStatus capture = hipStreamCaptureStatusActive;
-        if (hipStreamIsCapturing(st, &capture) != hipSuccess)
-        {
-            (void)hipGetLastError();
-            capture = hipStreamCaptureStatusActive; // unknown: treat as captured
-        }
-        const bool allow_route = !fork && capture == hipStreamCaptureStatusNone;
-        // A captured multi-batch call with a mixed-precision batch (the C5 hydro loop) ended in a
-        // host segmentation fault whose cause has not been shown; until it is, that combination
-        // is refused rather than run (the FP64 multi-batch capture is tested and stays enabled)
-        if (fork && capture != hipStreamCaptureStatusNone)
-            for (int bi = 0; bi < nb; ++bi)
-                if (batches[bi].n > 0 && batches[bi].precision == NNET_PRECISION_MIXED)
-                    return fail(c, NNET_E_STATE,
-                                "mixed precision in a multi-batch call under HIP graph capture is disabled");
-        if (fork)
-        {
-            HIP_TRY(c, hipEventRecord(c->fork_ev, st));
-            for (int k = 0; k < std::min(n_live, (int)nnet_ctx::NSIDE); ++k)
-                HIP_TRY(c, hipStreamWaitEvent(c->side[k], c->fork_ev, 0));
-        }
-        size_t order_off = 0, redo_off = 0;
-        int    live      = 0;
-        for (int bi = 0; bi < nb; ++bi)
-        {
-            const nnet_batch& b = batches[bi];
-            if (b.n == 0) continue;
-            const hipStream_t bst = fork ? c->side[live++ % nnet_ctx::NSIDE] : st;
-            const int         d   = b.network->d;
-            EosDev            ed{};
-            ed.kind = b.eos;
-            ed.mu   = b.mu;
-            if (b.eos == NNET_EOS_HELMHOLTZ)
-            {
-                for (int z = 0; z < b.z_dim; ++z)
-                    ed.Z[z] = b.Z[z];
-                ed.zdim  = b.z_dim;
-                ed.grid  = c->d_grid;
-                ed.cells = c->d_cells;
-            }
-            BatchDev bd{};
-            bd.n = b.n, bd.rho = b.rho, bd.rho_m1 = b.rho_m1, bd.temp = b.temp, bd.dt = b.dt;
-            bd.status = b.status, bd.substeps = b.substeps, bd.newton = b.newton, bd.fp64_solves = b.fp64_solves;
-            for (int z = 0; z < d; ++z)
-                bd.Y[z] = b.Y[z];
-            const NetDev& nd  = b.network->dev;
-            unsigned int* ctr = c->d_queue + (size_t)nnet_ctx::NCTR * bi;
-            bd.queue          = ctr;
-            hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, bst, ctr, nnet_ctx::NCTR);
-            const bool mix = b.precision == NNET_PRECISION_MIXED;
-            if (mix)
-            {
-                bd.redo       = c->d_redo + redo_off;
-                bd.redo_count = ctr + 4;
-                redo_off += b.n;
-            }
-            // ordered dispatch: longest first by the cost hint; in a mixed batch, the particles with
-            // a nonzero fp64 hint form the prefix [0, nH) of the order, the rest [nH, n)
-            const uint32_t*     route = (mix && allow_route) ? b.fp64_hint : nullptr;
-            const unsigned int* range = nullptr;
-            bd.order                  = nullptr;
-            if (b.cost_hint || route)
-            {
-                unsigned int* bins  = c->d_bins + (size_t)BIN_STRIDE * bi;
-                uint32_t*     order = c->d_order + order_off;
-                order_off += b.n;
-                const unsigned blocks = (unsigned)std::min<size_t>((b.n + 255) / 256, 2048);
-                hipLaunchKernelGGL(nnet_queue_reset, dim3((BIN_STRIDE + 255) / 256), dim3(256), 0, bst, bins,
-                                   BIN_STRIDE);
-                hipLaunchKernelGGL(nnet_order_count, dim3(blocks), dim3(256), 0, bst, b.cost_hint, route, b.n, bins);
-                hipLaunchKernelGGL(nnet_order_offsets, dim3(1), dim3(64), 0, bst, bins);
-                hipLaunchKernelGGL(nnet_order_scatter, dim3(blocks), dim3(256), 0, bst, b.cost_hint, route, b.n, bins,
-                                   order);
-                bd.order = order;
-                range    = bins + 2 * NKEY; // {0, nH, n}
-            }
-            const int         kind = b.network->kind;
-            const int         ak   = fork ? (live - 1) % nnet_ctx::NSIDE : 0;
-            const hipStream_t ast  = c->aux[ak];
-            hipError_t        herr = hipSuccess;
-            auto              chk  = [&](hipError_t e)
-            {
-                if (e != hipSuccess && herr == hipSuccess) herr = e;
-            };
-            auto              launch = [&](auto kern, int nt, const BatchDev& bdv, const StepParams& spv, hipStream_t s)
-            {
-                int per_cu = 0;
-                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, 0) != hipSuccess || per_cu < 1)
-                    per_cu = 1;
-                const size_t wgs = std::min<size_t>(b.n, (size_t)per_cu * c->n_cu);
-                hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(nt), 0, s, nd, ed, bdv, spv);
-            };
-            // mixed batch: the routed particles [0, nH) go to an FP64 pass on the aux stream while
-            // the mixed pass takes [nH, n); the particles the mixed pass gives up go to a last FP64
-            // pass over its redo list (rho_m1 already updated by the mixed pass)
-            BatchDev routed_bd  = bd;
-            routed_bd.queue     = ctr + 1;
-            routed_bd.qrange    = range;
-            routed_bd.fp64_pass = 1;
-            BatchDev mixed_bd   = bd;
-            mixed_bd.qrange     = route ? range + 1 : nullptr;
-            BatchDev redo_bd    = bd;
-            redo_bd.queue       = ctr + 2;
-            redo_bd.order       = bd.redo;
-            redo_bd.qrange      = ctr + 3; // {0, redo count}
-            redo_bd.fp64_pass   = 1;
-            StepParams redo_sp  = sp;
-            if (sp.use_drhodt == NNET_DRHODT_CPU) redo_sp.use_drhodt = NNET_DRHODT_OFF; // rho_m1 already updated
-            auto dispatch       = [&](auto netc, auto eosc)
-            {
-                constexpr int NV = decltype(netc)::value, EV = decltype(eosc)::value;
-                if (!mix)
-                {
-                    launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, bd, sp, bst);
-                    return;
-                }
-                if (route)
-                {
-                    chk(hipEventRecord(c->aux_fork[ak], bst));
-                    chk(hipStreamWaitEvent(ast, c->aux_fork[ak], 0));
-                    launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, routed_bd, sp, ast);
-                }
-                launch(nnet_solve_kernel<NV, EV, true>, Cfg<NV>::NT, mixed_bd, sp, bst);
-                launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, redo_bd, redo_sp, bst);
-                if (route)
-                {
-                    chk(hipEventRecord(c->aux_join[ak], ast));
-                    chk(hipStreamWaitEvent(bst, c->aux_join[ak], 0));
-                }
-            };
-            auto dispatch_eos = [&](auto netc)
-            {
-                if (b.eos == NNET_EOS_IDEAL) dispatch(netc, std::integral_constant<int, NNET_EOS_IDEAL>{});
-                else
-                    dispatch(netc, std::integral_constant<int, NNET_EOS_HELMHOLTZ>{});
-            };
-            if (kind == 14) dispatch_eos(std::integral_constant<int, 14>{});
-            else if (kind == 86)
-                dispatch_eos(std::integral_constant<int, 86>{});
-            else
-                dispatch_eos(std::integral_constant<int, 87>{});
-            HIP_TRY(c, herr);
-            HIP_TRY(c, hipGetLastError());
-        }
-        if (fork)
-            for (int k = 0; k < std::min(n_live, (int)nnet_ctx::NSIDE); ++k)
-            {
-                HIP_TRY(c, hipEventRecord(c->side_ev[k], c->side[k]));
-                HIP_TRY(c, hipStreamWaitEvent(st, c->side_ev[k], 0));
-            }
-        return NNET_OK;
+        return enqueue_compute(c, batches, nb, hydro_dt, previous_dt, use_drhodt, st, n_live,
+                               CallBufs{c->d_queue, c->d_bins, c->d_order, c->d_redo});
     }
 
     int nnet_compute_sync(nnet_ctx* c, const nnet_batch* batches, int nb, double hydro_dt, double previous_dt,

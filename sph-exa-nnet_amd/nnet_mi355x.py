@@ -73,6 +73,7 @@ def lib():
         L.nnet_helm_eos.argtypes = eos_args + [vp]
         L.nnet_helm_eos_sync.argtypes = eos_args
         L.nnet_helm_table_read.argtypes = [C.c_char_p, dp]
+        L.nnet_solve_slots.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip]
         _lib = L
     return _lib
 
@@ -81,7 +82,7 @@ EXPORTS = [
     "nnet_ctx_create", "nnet_ctx_destroy", "nnet_last_error", "nnet_ctx_load_helmholtz",
     "nnet_network_create", "nnet_network_builtin", "nnet_network_destroy", "nnet_network_reactions",
     "nnet_compute", "nnet_compute_sync", "nnet_ctx_set_max_iterations", "nnet_helm_eos", "nnet_helm_eos_sync",
-    "nnet_helm_table_read",
+    "nnet_helm_table_read", "nnet_solve_slots",
 ]
 
 
@@ -134,6 +135,12 @@ class Context:
         P = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
         self.check(lib().nnet_network_reactions(h, None, None, P(rb), P(pb), P(ids), P(nums)), "reactions")
         return rb, pb, ids, nums
+
+    def resident_slots(self, net, eos=EOS_HELMHOLTZ, precision=PRECISION_FP64):
+        """Particles one solve launch keeps in flight (resident workgroups of the persistent kernel)."""
+        out = C.c_int()
+        self.check(lib().nnet_solve_slots(self.h, net, eos, precision, C.byref(out)), "nnet_solve_slots")
+        return out.value
 
     def set_max_iterations(self, m):
         self.check(lib().nnet_ctx_set_max_iterations(self.h, m), "set_max_iterations")

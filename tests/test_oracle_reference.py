@@ -82,3 +82,27 @@ def test_particles_with_drho_dt(oracle_lib, ref_lib, net, eos):
         assert np.array_equal(a[3][:4], b[3][:4]) and same(a[4], b[4]), k
         checked += 1
     assert checked >= 8
+
+
+@pytest.mark.parametrize("cid,n", [(2, 384), (3, 256), (4, 48)])
+def test_reference_guarded_batch_equals_oracle_batch(oracle_lib, ref_lib, cid, n):
+    """The CPU baseline's reference batch (the reference's compiled solver in its OpenMP loop,
+    with the replay detection and cap) gives the oracle batch's results bit for bit."""
+    import os
+    import workloads
+    if not ref_lib.has_helmholtz:
+        pytest.skip("reference checker build without the Helmholtz EOS")
+    table = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sph-exa-nnet_amd", "data",
+                         "helm_table.bin")
+    oracle_lib.helm_load(table)
+    cfg = workloads.config(cid, n=n, seed=3300 + cid)
+    Yo = np.ascontiguousarray(cfg["Y"].copy())
+    To, dto = cfg["temp"].copy(), cfg["dt"].copy()
+    so, _, no = oracle_lib.batch(cfg["net"], cfg["rho"].copy(), To, Yo, dto, cfg["hydro_dt"], eos_kind=1, Z=cfg["Z"],
+                                 max_iter=3000)
+    Yr = np.ascontiguousarray(cfg["Y"].copy())
+    Tr, dtr = cfg["temp"].copy(), cfg["dt"].copy()
+    sr, nr = ref_lib.batch_guarded(cfg["net"], cfg["rho"].copy(), Tr, Yr, dtr, cfg["hydro_dt"], eos_kind=1, Z=cfg["Z"],
+                                   max_iter=3000)
+    assert np.array_equal(so, sr) and np.array_equal(no, nr)
+    assert np.array_equal(Yo, Yr) and np.array_equal(To, Tr) and np.array_equal(dto, dtr)
