@@ -301,6 +301,250 @@ static void gauss_solve(double* M, double* RHS, double* X, int n, double eps)
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Models of mixed-precision linear solves (test infrastructure): the device's mixed path (FP32
+// unpivoted LU in the reference's elimination order, FP64 residuals, LU iterative refinement) and
+// an alternative weighed against it (GMRES-IR: FP64 GMRES on the Ruiz-equilibrated system, left
+// preconditioned by its FP32 LU factors; Carson & Higham, SIAM J. Sci. Comput. 40(2), 2018).
+// mode 0 is the reference's FP64 eigen::solve; with mode 1 or 2 an iteration whose correction does
+// not converge (every |d_i| <= tol (|x_i| + s_i), s = (T, Y)) abandons the particle, which the batch
+// then integrates again in FP64 from the start of the step -- the device's redo pass.
+struct SolverModel
+{
+    int    mode   = 0;
+    double tol    = 1e-11;
+    int    sweeps = 1; // refinement sweeps (mode 1), outer iterations (mode 2)
+    int    inner  = 8; // GMRES iterations per outer iteration (mode 2)
+};
+static SolverModel& solver_model()
+{
+    static SolverModel m;
+    return m;
+}
+struct SolverCounts
+{
+    long solves = 0, converged = 0, inner = 0;
+};
+static SolverCounts& solver_counts()
+{
+    static SolverCounts c;
+    return c;
+}
+// set while a batch integrates an abandoned particle again in FP64
+static thread_local bool force_fp64 = false;
+
+// unpivoted LU of the column-major n x n matrix A (float), reference order and 1e-40 skip:
+// unit-diagonal U above the diagonal, multipliers L below, reciprocal pivots in rinv
+static void lu32(std::vector<float>& A, std::vector<float>& rinv, int n)
+{
+    const float eps = (float)k::epsilonSystem;
+    for (int i = 0; i < n; ++i)
+    {
+        for (int j = 0; j < i; ++j)
+        {
+            const float w = A[i + n * j];
+            if (!(std::abs(w) > eps))
+            {
+                A[i + n * j] = 0;
+                continue;
+            }
+            for (int c = j + 1; c < n; ++c)
+                A[i + n * c] -= w * A[j + n * c];
+        }
+        rinv[i] = 1.f / A[i + n * i];
+        for (int c = i + 1; c < n; ++c)
+            A[i + n * c] *= rinv[i];
+    }
+}
+// x = (LU)^-1 b in FP32
+static void lu32_solve(const std::vector<float>& A, const std::vector<float>& rinv, int n, const double* b, double* x)
+{
+    std::vector<float> y(n);
+    for (int i = 0; i < n; ++i)
+    {
+        float v = (float)b[i];
+        for (int j = 0; j < i; ++j)
+            v -= A[i + n * j] * y[j];
+        y[i] = v * rinv[i];
+    }
+    for (int i = n - 1; i >= 0; --i)
+    {
+        float v = y[i];
+        for (int j = i + 1; j < n; ++j)
+            v -= A[i + n * j] * y[j];
+        y[i] = v;
+    }
+    for (int i = 0; i < n; ++i)
+        x[i] = (double)y[i];
+}
+static void residual(const double* M, const double* b, const double* x, double* r, int n)
+{
+    for (int i = 0; i < n; ++i)
+    {
+        double v = b[i];
+        for (int j = 0; j < n; ++j)
+            v = std::fma(-M[i + n * j], x[j], v);
+        r[i] = v;
+    }
+}
+static bool small_correction(const double* d, const double* x, const double* s, int n, double tol)
+{
+    for (int i = 0; i < n; ++i)
+        if (!(std::abs(d[i]) <= tol * (std::abs(x[i]) + s[i] + 1e-30))) return false;
+    return true;
+}
+
+// The mixed solve of M x = b (M column-major, unmodified); s: the magnitude scale of each
+// unknown.  Returns false when the correction does not converge.
+static bool mixed_solve(const double* M, const double* b, double* x, const double* s, int n)
+{
+    const SolverModel& m = solver_model();
+    std::vector<double> r(n), dx(n), xn(n);
+    std::vector<float>  A(n * n), rinv(n);
+    if (m.mode == 1)
+    {
+        for (int i = 0; i < n * n; ++i)
+            A[i] = (float)M[i];
+        lu32(A, rinv, n);
+        lu32_solve(A, rinv, n, b, x);
+        for (int sw = 0; sw < m.sweeps; ++sw)
+        {
+            residual(M, b, x, r.data(), n);
+            lu32_solve(A, rinv, n, r.data(), dx.data());
+            for (int i = 0; i < n; ++i)
+                xn[i] = x[i] + dx[i];
+            const bool ok = small_correction(dx.data(), xn.data(), s, n, m.tol);
+            std::copy(xn.begin(), xn.end(), x);
+            if (ok) return true;
+        }
+        return false;
+    }
+    // GMRES-IR on the Ruiz-equilibrated system D_r M D_c (power-of-two scalings)
+    std::vector<double> dr(n, 1.), dc(n, 1.), S(M, M + n * n);
+    for (int it = 0; it < 8; ++it)
+    {
+        for (int i = 0; i < n; ++i)
+        {
+            double a = 0;
+            for (int j = 0; j < n; ++j)
+                a = std::max(a, std::abs(S[i + n * j]));
+            const double f = a > 0 ? std::ldexp(1., -std::ilogb(a) / 2) : 1.;
+            dr[i] *= f;
+            for (int j = 0; j < n; ++j)
+                S[i + n * j] *= f;
+        }
+        for (int j = 0; j < n; ++j)
+        {
+            double a = 0;
+            for (int i = 0; i < n; ++i)
+                a = std::max(a, std::abs(S[i + n * j]));
+            const double f = a > 0 ? std::ldexp(1., -std::ilogb(a) / 2) : 1.;
+            dc[j] *= f;
+            for (int i = 0; i < n; ++i)
+                S[i + n * j] *= f;
+        }
+    }
+    for (int i = 0; i < n * n; ++i)
+        A[i] = (float)S[i];
+    lu32(A, rinv, n);
+    auto precond = [&](const double* v, double* out) { lu32_solve(A, rinv, n, v, out); };
+    // x0 = D_c (LU)^-1 D_r b
+    {
+        std::vector<double> t(n);
+        for (int i = 0; i < n; ++i)
+            t[i] = dr[i] * b[i];
+        precond(t.data(), x);
+        for (int i = 0; i < n; ++i)
+            x[i] *= dc[i];
+    }
+    const int           K = m.inner;
+    std::vector<double> V((size_t)(K + 1) * n), H((size_t)(K + 1) * K, 0.), g(K + 1), cs(K), sn(K), y(K), w(n), t(n);
+    for (int outer = 0; outer < m.sweeps; ++outer)
+    {
+        residual(M, b, x, r.data(), n);
+        for (int i = 0; i < n; ++i)
+            t[i] = dr[i] * r[i];
+        precond(t.data(), w.data()); // z0 = (LU)^-1 D_r r
+        double beta = 0;
+        for (int i = 0; i < n; ++i)
+            beta += w[i] * w[i];
+        beta = std::sqrt(beta);
+        std::fill(dx.begin(), dx.end(), 0.);
+        if (beta > 0)
+        {
+            for (int i = 0; i < n; ++i)
+                V[i] = w[i] / beta;
+            std::fill(g.begin(), g.end(), 0.);
+            g[0]  = beta;
+            int k = 0;
+            for (; k < K; ++k)
+            {
+                // w = (LU)^-1 S v_k
+                for (int i = 0; i < n; ++i)
+                {
+                    double v = 0;
+                    for (int j = 0; j < n; ++j)
+                        v += S[i + n * j] * V[(size_t)k * n + j];
+                    t[i] = v;
+                }
+                precond(t.data(), w.data());
+                for (int q = 0; q <= k; ++q)
+                {
+                    double h = 0;
+                    for (int i = 0; i < n; ++i)
+                        h += w[i] * V[(size_t)q * n + i];
+                    H[(size_t)q * K + k] = h;
+                    for (int i = 0; i < n; ++i)
+                        w[i] -= h * V[(size_t)q * n + i];
+                }
+                double hn = 0;
+                for (int i = 0; i < n; ++i)
+                    hn += w[i] * w[i];
+                hn                         = std::sqrt(hn);
+                H[(size_t)(k + 1) * K + k] = hn;
+                for (int i = 0; i < n; ++i)
+                    V[(size_t)(k + 1) * n + i] = hn > 0 ? w[i] / hn : 0.;
+                for (int q = 0; q < k; ++q)
+                {
+                    const double a = H[(size_t)q * K + k], c = H[(size_t)(q + 1) * K + k];
+                    H[(size_t)q * K + k]       = cs[q] * a + sn[q] * c;
+                    H[(size_t)(q + 1) * K + k] = -sn[q] * a + cs[q] * c;
+                }
+                const double a = H[(size_t)k * K + k], c = H[(size_t)(k + 1) * K + k], rr = std::hypot(a, c);
+                cs[k]                      = rr > 0 ? a / rr : 1.;
+                sn[k]                      = rr > 0 ? c / rr : 0.;
+                H[(size_t)k * K + k]       = rr;
+                H[(size_t)(k + 1) * K + k] = 0;
+                g[k + 1]                   = -sn[k] * g[k];
+                g[k]                       = cs[k] * g[k];
+#pragma omp atomic
+                ++solver_counts().inner;
+                if (std::abs(g[k + 1]) <= 1e-8 * beta) {++k; break;}
+            }
+            for (int q = k - 1; q >= 0; --q)
+            {
+                double v = g[q];
+                for (int p = q + 1; p < k; ++p)
+                    v -= H[(size_t)q * K + p] * y[p];
+                y[q] = v / H[(size_t)q * K + q];
+            }
+            for (int i = 0; i < n; ++i)
+            {
+                double v = 0;
+                for (int q = 0; q < k; ++q)
+                    v += V[(size_t)q * n + i] * y[q];
+                dx[i] = dc[i] * v;
+            }
+        }
+        for (int i = 0; i < n; ++i)
+            xn[i] = x[i] + dx[i];
+        const bool ok = small_correction(dx.data(), xn.data(), s, n, m.tol);
+        std::copy(xn.begin(), xn.end(), x);
+        if (ok) return true;
+    }
+    return false;
+}
+
 struct Work
 {
     std::vector<double> Mp, RHS, DY, rates, Ybuf;
@@ -318,6 +562,7 @@ struct ParticleStats
 {
     uint32_t substeps = 0, newton = 0, rejections = 0, max_run = 0, run = 0;
     uint8_t  status   = ST_OK;
+    bool     abandoned = false; // a mixed solve did not converge (solver model 1 or 2)
 };
 
 // Optional per-iteration trace, for golden-vector fixtures.
@@ -384,7 +629,25 @@ static ParticleStats solve_particle(int net, int d, const EosSpec& eosSpec, Work
             st.status = ST_ZERO_DT;
             return st;
         }
-        gauss_solve(w.Mp.data(), w.RHS.data(), w.DY.data(), d + 1, k::epsilonSystem);
+        if (solver_model().mode == 0 || force_fp64)
+            gauss_solve(w.Mp.data(), w.RHS.data(), w.DY.data(), d + 1, k::epsilonSystem);
+        else
+        {
+            std::vector<double> sc(d + 1);
+            sc[0] = T;
+            for (int j = 0; j < d; ++j)
+                sc[j + 1] = std::abs(Y[j]);
+            const bool ok = mixed_solve(w.Mp.data(), w.RHS.data(), w.DY.data(), sc.data(), d + 1);
+#pragma omp atomic
+            ++solver_counts().solves;
+            if (!ok)
+            {
+                st.abandoned = true;
+                return st;
+            }
+#pragma omp atomic
+            ++solver_counts().converged;
+        }
         ++st.newton;
         ++total_iter;
 
@@ -658,8 +921,9 @@ extern "C"
     void oracle_batch(int net, int eos_kind, double mu, const double* Z, int zdim, size_t n, double* rho,
                       double* rho_m1, double** Y, double* temp, double* dt, double hydro_dt, double previous_dt,
                       int use_drhodt, uint8_t* status, uint32_t* substeps, uint32_t* newton, int nthreads,
-                      long max_iter, uint32_t* max_run)
+                      long max_iter, uint32_t* max_run, uint8_t* redone)
     {
+        if (redone) std::fill(redone, redone + n, (uint8_t)0);
         const int d  = dimension_of(net);
         const int nR = reactions_for(net).size();
         if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -696,8 +960,21 @@ extern "C"
                     {
                         if (rho_m1[i] != 0) drho_dt = (rho[i] - rho_m1[i]) / previous_dt;
                     }
+                    const double T0 = temp[i], dt0 = dt[i];
                     st = solve_particle(net, d, e, w, Yl.data(), temp[i], rho[i], drho_dt, hydro_dt, dt[i], max_iter,
                                         nullptr);
+                    if (st.abandoned)
+                    {
+                        // the device's redo pass: the whole step again in FP64
+                        for (int j = 0; j < d; ++j)
+                            Yl[j] = Y[j][i];
+                        temp[i] = T0, dt[i] = dt0;
+                        force_fp64 = true;
+                        st = solve_particle(net, d, e, w, Yl.data(), temp[i], rho[i], drho_dt, hydro_dt, dt[i],
+                                            max_iter, nullptr);
+                        force_fp64 = false;
+                        if (redone) redone[i] = 1;
+                    }
                     for (int j = 0; j < d; ++j)
                         Y[j][i] = Yl[j];
                 }
@@ -716,4 +993,20 @@ extern "C"
     // Test-only sensitivity probe: 1 moves a deterministic 3/8 of the rate functions'
     // exp/log/pow results by one ulp; 0 restores the exact restatement.
     void oracle_set_ulp_perturbation(int on) { ulp_perturbation() = on; }
+
+    // Linear-solve model of the batch path: 0 the reference's FP64 eigen::solve; 1 FP32 LU with
+    // `sweeps` FP64-residual refinement sweeps; 2 GMRES-IR with `sweeps` outer and `inner` GMRES
+    // iterations; tol is the correction tolerance.  Counts: solves, converged solves, GMRES
+    // iterations (reset by this call).
+    void oracle_set_solver_model(int mode, double tol, int sweeps, int inner)
+    {
+        solver_model()  = SolverModel{mode, tol, sweeps, inner};
+        solver_counts() = SolverCounts{};
+    }
+    void oracle_solver_counts(long* out)
+    {
+        out[0] = solver_counts().solves;
+        out[1] = solver_counts().converged;
+        out[2] = solver_counts().inner;
+    }
 }

@@ -128,7 +128,10 @@ class Oracle(_Lib):
         L.oracle_helm_raw.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, D]
         L.oracle_batch.argtypes = [C.c_int, C.c_int, C.c_double, D, C.c_int, C.c_size_t, D, D, C.POINTER(D), D, D,
                                    C.c_double, C.c_double, C.c_int, C.POINTER(C.c_uint8), C.POINTER(C.c_uint32),
-                                   C.POINTER(C.c_uint32), C.c_int, C.c_long, C.POINTER(C.c_uint32)]
+                                   C.POINTER(C.c_uint32), C.c_int, C.c_long, C.POINTER(C.c_uint32),
+                                   C.POINTER(C.c_uint8)]
+        L.oracle_set_solver_model.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int]
+        L.oracle_solver_counts.argtypes = [C.POINTER(C.c_long)]
         L.oracle_dynamic_batch_size.argtypes = [C.c_size_t, C.c_int]
 
     def helm_load(self, path):
@@ -154,15 +157,30 @@ class Oracle(_Lib):
         sub = np.zeros(n, np.uint32)
         newt = np.zeros(n, np.uint32)
         mrun = np.zeros(n, np.uint32)
+        redone = np.zeros(n, np.uint8)
         rm1 = rho_m1 if rho_m1 is not None else rho.copy()
         Zc = np.ascontiguousarray(Z if Z is not None else np.zeros(1), dtype=np.float64)
         self.L.oracle_batch(net, eos_kind, mu, _p(Zc), len(Zc) if Z is not None else 0, n, _p(rho), _p(rm1), rows,
                             _p(temp), _p(dt), hydro_dt, hydro_dt if previous_dt is None else previous_dt,
                             int(use_drhodt), status.ctypes.data_as(C.POINTER(C.c_uint8)),
                             sub.ctypes.data_as(C.POINTER(C.c_uint32)), newt.ctypes.data_as(C.POINTER(C.c_uint32)),
-                            nthreads, max_iter, mrun.ctypes.data_as(C.POINTER(C.c_uint32)))
+                            nthreads, max_iter, mrun.ctypes.data_as(C.POINTER(C.c_uint32)),
+                            redone.ctypes.data_as(C.POINTER(C.c_uint8)))
         self.last_max_run = mrun
+        self.last_redone = redone
         return status, sub, newt
+
+
+    def set_solver_model(self, mode, tol=1e-11, sweeps=1, inner=8):
+        """Linear-solve model of batch(): 0 FP64 (the reference), 1 FP32 LU + FP64 refinement (the
+        device's mixed path), 2 GMRES-IR; a particle whose mixed solve does not converge is
+        integrated again in FP64 (self.last_redone marks it)."""
+        self.L.oracle_set_solver_model(mode, tol, sweeps, inner)
+
+    def solver_counts(self):
+        out = (C.c_long * 3)()
+        self.L.oracle_solver_counts(out)
+        return {"solves": out[0], "converged": out[1], "gmres_iterations": out[2]}
 
 
 class Reference(_Lib):

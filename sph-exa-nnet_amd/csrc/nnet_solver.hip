@@ -1702,6 +1702,31 @@ static int check_call(nnet_ctx* c, const nnet_batch* batches, int nb, double hyd
     return NNET_OK;
 }
 
+// Is `st` being captured into a HIP graph?  (An unknown status counts as captured.)
+static bool capturing(hipStream_t st)
+{
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusActive;
+    if (hipStreamIsCapturing(st, &capture) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return true;
+    }
+    return capture != hipStreamCaptureStatusNone;
+}
+
+// Checks a call makes before it touches the stream when the stream is being captured.  A
+// captured multi-batch call with a mixed-precision batch (the C5 hydro loop) ended in a host
+// segmentation fault whose cause has not been shown; until it is, that combination is refused
+// rather than run (the FP64 multi-batch capture is tested and stays enabled).
+static int check_capture(nnet_ctx* c, const nnet_batch* batches, int nb, int n_live)
+{
+    if (n_live > 1)
+        for (int bi = 0; bi < nb; ++bi)
+            if (batches[bi].n > 0 && batches[bi].precision == NNET_PRECISION_MIXED)
+                return fail(c, NNET_E_STATE, "mixed precision in a multi-batch call under HIP graph capture is disabled");
+    return NNET_OK;
+}
+
 // Enqueues the kernels of one nnet_compute call on `st` with the given buffers.
 static int enqueue_compute(nnet_ctx* c, const nnet_batch* batches, int nb, double hydro_dt, double previous_dt,
                            int use_drhodt, hipStream_t st, int n_live, const CallBufs& bufs)
@@ -1710,21 +1735,7 @@ static int enqueue_compute(nnet_ctx* c, const nnet_batch* batches, int nb, doubl
     const bool fork = n_live > 1;
     // the fp64 routing hint forks an auxiliary stream per batch; it is honoured in single-batch
     // calls outside HIP graph capture and ignored otherwise (results stay valid without it)
-    hipStreamCaptureStatus capture = hipStreamCaptureStatusActive;
-    if (hipStreamIsCapturing(st, &capture) != hipSuccess)
-    {
-        (void)hipGetLastError();
-        capture = hipStreamCaptureStatusActive; // unknown: treat as captured
-    }
-    const bool allow_route = !fork && capture == hipStreamCaptureStatusNone;
-    // A captured multi-batch call with a mixed-precision batch (the C5 hydro loop) ended in a
-    // host segmentation fault whose cause has not been shown; until it is, that combination
-    // is refused rather than run (the FP64 multi-batch capture is tested and stays enabled)
-    if (fork && capture != hipStreamCaptureStatusNone)
-        for (int bi = 0; bi < nb; ++bi)
-            if (batches[bi].n > 0 && batches[bi].precision == NNET_PRECISION_MIXED)
-                return fail(c, NNET_E_STATE,
-                            "mixed precision in a multi-batch call under HIP graph capture is disabled");
+    const bool allow_route = !fork && !capturing(st);
     if (fork)
     {
         HIP_TRY(c, hipEventRecord(c->fork_ev, st));
@@ -2165,6 +2176,16 @@ extern "C"
         HIP_TRY(c, hipSetDevice(c->device));
         hipStream_t st   = (hipStream_t)stream;
         const Needs need = call_needs(batches, nb);
+        const bool  grow = nb > c->n_queue || need.order > c->n_order || (need.order > 0 && nb > c->n_bins) ||
+                          need.redo > c->n_redo;
+        if (capturing(st))
+        {
+            // nothing that would invalidate the caller's capture (a synchronisation, an allocation)
+            if ((rc = check_capture(c, batches, nb, n_live)) != NNET_OK) return rc;
+            if (grow)
+                return fail(c, NNET_E_STATE, "the library's buffers must grow for this call: make the same call once "
+                                             "outside HIP graph capture first");
+        }
         if (nb > c->n_queue)
         {
             // grows only in calls with more batches than before (a capture-safe caller keeps nb fixed)
