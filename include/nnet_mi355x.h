@@ -151,6 +151,20 @@ typedef struct nnet_batch {
 int nnet_compute(nnet_ctx* ctx, const nnet_batch* batches, int nb, double hydro_dt,
                  double previous_dt, int use_drhodt, void* stream);
 
+/* nnet_compute without any allocation: the same computation, with the library's work-queue
+ * counters, ordering bins, dispatch order and redo lists placed in a fixed device scratch array
+ * instead of buffers the library grows with hipMalloc.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The scratch is 64 KiB + 16 MiB per GPU, held from the first context on that
+ * GPU until the process ends; a call may hold at most 119 batches and at most 4,194,304 list
+ * entries -- one per particle of each batch with a cost_hint (or an honoured fp64_hint), plus one
+ * per particle of each mixed-precision batch -- and is refused with NNET_E_ARG beyond that, before
+ * anything is enqueued.  Every call on a GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (other streams, other threads, other contexts) race.  Same streams, capture rules and
+ * results as nnet_compute. */
+int nnet_compute_noalloc(nnet_ctx* ctx, const nnet_batch* batches, int nb, double hydro_dt,
+                         double previous_dt, int use_drhodt, void* stream);
+
 /* Cap on Newton iterations per particle per call (default 10000; 0 = no cap).  The reference
  * controller can also cycle between two rejected timesteps for thousands of attempts before (or
  * instead of) accepting one; the cap bounds that work.  A capped particle is left at its last

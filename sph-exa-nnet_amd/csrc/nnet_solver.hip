@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1537,6 +1538,8 @@ struct nnet_ctx
     hipEvent_t  aux_fork[NSIDE] = {};
     hipEvent_t  aux_join[NSIDE] = {};
     std::vector<void*> owned;
+    // this device's copy of the fixed scratch of nnet_compute_noalloc
+    uint32_t* scratch_ptr = nullptr;
 };
 
 struct nnet_network
@@ -1664,6 +1667,35 @@ static Needs call_needs(const nnet_batch* batches, int nb)
         if (b.precision == NNET_PRECISION_MIXED) n.redo += b.n;
     }
     return n;
+}
+
+// Fixed device scratch of nnet_compute_noalloc, one copy per GPU the library runs on, held until
+// the process ends: SCRATCH_CTR_WORDS words for the per-batch work-queue counters and ordering
+// bins (NCTR + BIN_STRIDE words per batch), then SCRATCH_LIST_WORDS words for the dispatch-order
+// and redo-list entries (one per particle of each cost-ordered batch, one per particle of each
+// mixed-precision batch).  64 KiB + 16 MiB.  Host code addresses it only through the pointer that
+// hipGetSymbolAddress returns for the current device (scratch_ptr_of), never by its name.
+constexpr size_t SCRATCH_CTR_WORDS = 16384, SCRATCH_LIST_WORDS = size_t(4) << 20;
+static __device__ __align__(256) uint32_t scratch[SCRATCH_CTR_WORDS + SCRATCH_LIST_WORDS];
+
+// Per-device address of `scratch`, looked up once per device with that device current.
+static std::mutex g_scratch_mu;
+static std::map<int, uint32_t*> g_scratch_ptr;
+static int scratch_ptr_of(int device, uint32_t** out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto                        it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end())
+    {
+        *out = it->second;
+        return NNET_OK;
+    }
+    void* p = nullptr;
+    if (hipSetDevice(device) != hipSuccess || hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+        return NNET_E_HIP;
+    g_scratch_ptr[device] = static_cast<uint32_t*>(p);
+    *out                  = static_cast<uint32_t*>(p);
+    return NNET_OK;
 }
 
 // Argument checks of nnet_compute; nothing is allocated or enqueued.  n_live: non-empty batches.
@@ -1899,7 +1931,8 @@ extern "C"
                 nnet_ctx_destroy(c);
                 return NNET_E_HIP;
             }
-        if (hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess)
+        if (hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+            scratch_ptr_of(device, &c->scratch_ptr) != NNET_OK)
         {
             nnet_ctx_destroy(c);
             return NNET_E_HIP;
@@ -2229,6 +2262,31 @@ extern "C"
         if (rc != NNET_OK) return rc;
         HIP_TRY(c, hipDeviceSynchronize());
         return NNET_OK;
+    }
+
+    int nnet_compute_noalloc(nnet_ctx* c, const nnet_batch* batches, int nb, double hydro_dt, double previous_dt,
+                             int use_drhodt, void* stream)
+    {
+        int n_live = 0;
+        int rc     = check_call(c, batches, nb, hydro_dt, use_drhodt, n_live);
+        if (rc != NNET_OK) return rc;
+        if (!c->scratch_ptr) return fail(c, NNET_E_STATE, "nnet_compute_noalloc: no fixed scratch on this device");
+        const Needs  need = call_needs(batches, nb);
+        const size_t ctr  = (size_t)(nnet_ctx::NCTR + BIN_STRIDE) * (size_t)nb;
+        if (ctr > SCRATCH_CTR_WORDS)
+            return fail(c, NNET_E_ARG, "nnet_compute_noalloc: at most " +
+                                           std::to_string(SCRATCH_CTR_WORDS / (nnet_ctx::NCTR + BIN_STRIDE)) +
+                                           " batches per call");
+        if (need.order + need.redo > SCRATCH_LIST_WORDS)
+            return fail(c, NNET_E_ARG, "nnet_compute_noalloc: at most " + std::to_string(SCRATCH_LIST_WORDS) +
+                                           " particles per call counting each cost-hinted and each mixed-precision "
+                                           "batch once (both: twice)");
+        HIP_TRY(c, hipSetDevice(c->device));
+        if (capturing((hipStream_t)stream) && (rc = check_capture(c, batches, nb, n_live)) != NNET_OK) return rc;
+        uint32_t* base = c->scratch_ptr;
+        CallBufs  bufs{base, base + (size_t)nnet_ctx::NCTR * nb, base + SCRATCH_CTR_WORDS,
+                      base + SCRATCH_CTR_WORDS + need.order};
+        return enqueue_compute(c, batches, nb, hydro_dt, previous_dt, use_drhodt, (hipStream_t)stream, n_live, bufs);
     }
 
     int nnet_helm_eos(nnet_ctx* c, size_t n, int dimension, const double* Z, const double* temp, const double* rho,

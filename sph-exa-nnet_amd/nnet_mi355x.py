@@ -69,6 +69,7 @@ def lib():
         L.nnet_network_reactions.argtypes = [vp, ip, ip, ip, ip, ip, ip]
         L.nnet_compute.argtypes = [vp, C.POINTER(Batch), C.c_int, C.c_double, C.c_double, C.c_int, vp]
         L.nnet_compute_sync.argtypes = [vp, C.POINTER(Batch), C.c_int, C.c_double, C.c_double, C.c_int]
+        L.nnet_compute_noalloc.argtypes = [vp, C.POINTER(Batch), C.c_int, C.c_double, C.c_double, C.c_int, vp]
         eos_args = [vp, C.c_size_t, C.c_int, dp, vp, vp, C.POINTER(vp), vp, vp, vp, vp, vp]
         L.nnet_helm_eos.argtypes = eos_args + [vp]
         L.nnet_helm_eos_sync.argtypes = eos_args
@@ -82,7 +83,7 @@ EXPORTS = [
     "nnet_ctx_create", "nnet_ctx_destroy", "nnet_last_error", "nnet_ctx_load_helmholtz",
     "nnet_network_create", "nnet_network_builtin", "nnet_network_destroy", "nnet_network_reactions",
     "nnet_compute", "nnet_compute_sync", "nnet_ctx_set_max_iterations", "nnet_helm_eos", "nnet_helm_eos_sync",
-    "nnet_helm_table_read", "nnet_solve_slots",
+    "nnet_helm_table_read", "nnet_solve_slots", "nnet_compute_noalloc",
 ]
 
 
