@@ -1357,6 +1357,22 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
     }
 }
 
+// The solve kernels are compiled in translation units of their own, one per network (this file
+// built with NNET_SOLVE_NET=<network>, which keeps only the code above), so the library's
+// objects build in parallel; the host part below launches them through these declarations.
+#define NNET_SOLVE_INSTANCES(SPEC, NET)                                                                      \
+    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_IDEAL, false>(NetDev, EosDev, BatchDev, StepParams);     \
+    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_IDEAL, true>(NetDev, EosDev, BatchDev, StepParams);      \
+    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_HELMHOLTZ, false>(NetDev, EosDev, BatchDev, StepParams); \
+    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_HELMHOLTZ, true>(NetDev, EosDev, BatchDev, StepParams);
+#ifdef NNET_SOLVE_NET
+NNET_SOLVE_INSTANCES(, NNET_SOLVE_NET)
+} // namespace nnet_amd
+#else
+NNET_SOLVE_INSTANCES(extern, 14)
+NNET_SOLVE_INSTANCES(extern, 86)
+NNET_SOLVE_INSTANCES(extern, 87)
+
 // ---------------------------------------------------------------------------------------------
 // Post-step EOS refresh (computeHelmholtzImpl, parallel_nuclear_net_impl.cpp:116-141): one lane
 // per particle.  The abundances are read species by species, each load coalesced across the
@@ -2327,3 +2343,4 @@ extern "C"
         return NNET_OK;
     }
 }
+#endif // NNET_SOLVE_NET

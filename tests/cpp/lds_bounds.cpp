@@ -198,6 +198,10 @@ void run()
         const long        uLimit = fp32 ? 2L * L::oBE : (long)L::oY;              // what sU may reach
         check(uEnd <= uLimit, t2 + "packed rows below the live arrays", uEnd, 0, uLimit + 1);
         if (fp32) check(L::NU + lEnd + riEnd <= 2L * L::oBE, t2 + "factors below the tail", L::NU + lEnd + riEnd, 0, 2L * L::oBE + 1);
+        // how often each packed-row / multiplier entry is stored (each must be stored exactly once)
+        std::vector<int> uStores(uEnd, 0), lStores(lEnd, 0);
+        auto             storeU = [&](long idx) { if (idx >= 0 && idx < uEnd) ++uStores[idx]; };
+        auto             storeL = [&](long idx) { if (idx >= 0 && idx < lEnd) ++lStores[idx]; };
         for (int step = 0; step < X::NS; ++step)
         {
             const int p0 = 4 * step, Tp = p0 >> 4, c0 = p0 & 15, v0 = c0 >> 2, nq = (N - p0) < 4 ? (N - p0) : 4;
@@ -222,12 +226,19 @@ void run()
                     for (int q = 0; q < nq; ++q)
                     {
                         for (int c = q + 1; c < 4; ++c)
-                            if (p0 + c <= N) IN(t2 + "sU diag", X::urow(p0 + q) + p0 + c, 0, uEnd);
+                            if (p0 + c <= N)
+                            {
+                                IN(t2 + "sU diag", X::urow(p0 + q) + p0 + c, 0, uEnd);
+                                storeU(X::urow(p0 + q) + p0 + c);
+                            }
                         if (fp32)
                         {
                             IN(t2 + "sRinv", p0 + q, 0, riEnd);
                             for (int c = 0; c < q; ++c)
+                            {
                                 IN(t2 + "sL diag", X::lrow(p0 + q) + p0 + c, 0, lEnd);
+                                storeL(X::lrow(p0 + q) + p0 + c);
+                            }
                         }
                     }
                 const int HALF = NT / 2;
@@ -238,7 +249,11 @@ void run()
                         for (int q = 0; q < 4; ++q)
                         {
                             IN(t2 + "row lane P", X::pcol(step, i, q), 0, panEnd);
-                            if (fp32 && q < nq) IN(t2 + "row lane sL", X::lrow(i) + p0 + q, 0, lEnd);
+                            if (fp32 && q < nq)
+                            {
+                                IN(t2 + "row lane sL", X::lrow(i) + p0 + q, 0, lEnd);
+                                storeL(X::lrow(i) + p0 + q);
+                            }
                         }
                 }
                 else
@@ -248,7 +263,11 @@ void run()
                         for (int q = 0; q < 4; ++q)
                         {
                             IN(t2 + "column lane RT", X::prow(step, j, q), 0, panEnd);
-                            if (q < nq) IN(t2 + "column lane sU", X::urow(p0 + q) + j, 0, uEnd);
+                            if (q < nq)
+                            {
+                                IN(t2 + "column lane sU", X::urow(p0 + q) + j, 0, uEnd);
+                                storeU(X::urow(p0 + q) + j);
+                            }
                         }
                 }
                 // MFMA operands: (i in range) ? -P(i, lr) : 0 and (j in range) ? RT(j, lr) : 0
@@ -268,6 +287,15 @@ void run()
                 }
             }
         }
+        // every normalised-row entry (k, j), k < j <= N, and (mixed) every multiplier (i, c), c < i,
+        // is stored by exactly one lane of exactly one block
+        for (int kk = 0; kk < N; ++kk)
+            for (int j = kk + 1; j <= N; ++j)
+                check(uStores[X::urow(kk) + j] == 1, t2 + "sU entry stored once", X::urow(kk) + j, 1, 2);
+        if (fp32)
+            for (int i = 1; i < N; ++i)
+                for (int c = 0; c < i; ++c)
+                    check(lStores[X::lrow(i) + c] == 1, t2 + "sL entry stored once", X::lrow(i) + c, 1, 2);
         // back substitution: right-hand side column, then the readlane chain's U loads
         for (int lane = 0; lane < 64; ++lane)
         {

@@ -26,18 +26,28 @@ def _functions(tmp_path):
             pytest.skip(f"{tool} not available")
     if not os.path.exists(LIB):
         pytest.skip("libnnet_mi355x.so not built")
-    fb, co = str(tmp_path / "fb.bin"), str(tmp_path / "co.o")
+    fb = str(tmp_path / "fb.bin")
     subprocess.run([os.path.join(LLVM, "llvm-objcopy"), f"--dump-section=.hip_fatbin={fb}", LIB, str(tmp_path / "x")],
                    check=True)
-    subprocess.run([os.path.join(LLVM, "clang-offload-bundler"), "--unbundle", "--type=o", f"--input={fb}",
-                    "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={co}"], check=True)
-    out = subprocess.run([os.path.join(LLVM, "llvm-readelf"), "-sW", co], check=True, capture_output=True,
-                         text=True).stdout
+    # one offload bundle per object of the library (the host part and the solve kernels of each
+    # network are separate objects); each holds its own gfx950 code object
+    blob = open(fb, "rb").read()
+    magic = b"__CLANG_OFFLOAD_BUNDLE__"
+    starts = [m.start() for m in re.finditer(re.escape(magic), blob)]
+    assert starts, "no offload bundle in the library"
     funcs = {}
-    for line in out.splitlines():
-        f = line.split()
-        if len(f) >= 8 and f[3] == "FUNC":
-            funcs[f[7]] = (int(f[2]), f[4])
+    for k, s in enumerate(starts):
+        part, co = str(tmp_path / f"fb{k}.bin"), str(tmp_path / f"co{k}.o")
+        with open(part, "wb") as f:
+            f.write(blob[s:starts[k + 1] if k + 1 < len(starts) else len(blob)])
+        subprocess.run([os.path.join(LLVM, "clang-offload-bundler"), "--unbundle", "--type=o", f"--input={part}",
+                        "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={co}"], check=True)
+        out = subprocess.run([os.path.join(LLVM, "llvm-readelf"), "-sW", co], check=True, capture_output=True,
+                             text=True).stdout
+        for line in out.splitlines():
+            f = line.split()
+            if len(f) >= 8 and f[3] == "FUNC":
+                funcs[f[7]] = (int(f[2]), f[4])
     return funcs
 
 
