@@ -137,6 +137,8 @@ def run_c5(args, torch, nm):
     parts, dts = workloads.config5(n=args.n, steps=args.steps)
     prec = nm.PRECISION_FP64
     ctx = nm.Context(0, helm_table=TABLE)
+    if args.per_batch_launches:
+        ctx.set_fused_launch(False)
     sts = []
     for c in parts:
         _, st = device_config_from(torch, c)
@@ -183,6 +185,7 @@ def run_c5(args, torch, nm):
         "data": "synthetic",
         "config": {"workload": "mixed-network C5 (net14 + net86 + net87, one call per step, HIP graph)",
                    "particles": [int(c["rho"].size) for c in parts], "precision": args.precision,
+                   "launch": "one fused launch per step" if ctx.fused_launch() else "one launch per batch",
                    "hydro_dt_range": [float(dts.min()), float(dts.max())],
                    "last_step_substeps_hist_bins": ["0", "1", "2-3", "4-7", "8-15", "16-31", "32-63", ">=64"],
                    "last_step_substeps_hist": hist.tolist()},
@@ -302,6 +305,8 @@ def main():
                     help="Newton-iteration cap per particle and step (nnet_ctx_set_max_iterations; the oracle and "
                          "the CPU baseline use the same cap)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--per-batch-launches", action="store_true",
+                    help="--config 5: one launch per batch (side streams) instead of the fused launch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times): the reference kernel whose "
                          "FETCH_SIZE/WRITE_SIZE counters calibrate the HBM traffic counters")

@@ -173,9 +173,45 @@ int nnet_ctx_set_max_iterations(nnet_ctx* ctx, long max_iter);
 
 /* Particles a solve launch integrates concurrently on this context's device: the workgroups of
  * the persistent kernel for (rates_kind, eos, precision) that stay resident (occupancy per CU x
- * CUs), one particle each.  No counterpart in the reference (whose launch is one thread per
- * particle, parallel_nuclear_net.cu:144-156); for tools that model or tune the launch. */
+ * CUs) times the particles each integrates at once.  No counterpart in the reference (whose
+ * launch is one thread per particle, parallel_nuclear_net.cu:144-156); for tools that model or
+ * tune the launch. */
 int nnet_solve_slots(nnet_ctx* ctx, int rates_kind, int eos, int precision, int* slots);
+
+/* Particles per workgroup of the solve launches for (rates_kind, eos, precision): net14 (one
+ * wavefront per particle) 1, 2 or 4 (default 1), net86 / net87 (four wavefronts per particle)
+ * 1.  Results do not depend on it.  Replaces the reference's compile-time launch shape
+ * CUDA_NUM_THREAD_PER_BLOCK_NNET, particles per thread block
+ * there (parallel/CUDA/parallel_nuclear_net.cuh:51-64); the autotuner
+ * (sph-exa-nnet_amd/autotune.py) measures the choices and writes them to its table. */
+int nnet_ctx_set_particles_per_workgroup(nnet_ctx* ctx, int rates_kind, int eos, int precision, int ppw);
+int nnet_ctx_get_particles_per_workgroup(nnet_ctx* ctx, int rates_kind, int eos, int precision, int* ppw);
+
+/* Calls whose batches belong to several networks (the reference makes one call per network,
+ * parallel_nuclear_net.hpp:86-90): fused = 1 (default) runs them as one persistent launch whose
+ * workgroups drain the batches' work queues in turn (net87, net86, then net14 at four particles
+ * per workgroup) when the batches share the EOS and the precision, number at most 4 and hold at
+ * most 192 species and 192 charges in all; otherwise, and with fused = 0, each batch is its own
+ * launch on a side stream.  Results do not depend on it.  Chosen by the autotuner. */
+int nnet_ctx_set_fused_launch(nnet_ctx* ctx, int fused);
+int nnet_ctx_get_fused_launch(nnet_ctx* ctx, int* fused);
+
+/* The launch parameters above as one record, and the autotuner's table that holds them: a text
+ * file of lines "ppw <rates_kind> <eos> <precision> <particles per workgroup>" and
+ * "fused <0|1>" ('#' starts a comment; settings not listed keep their defaults).
+ * sph-exa-nnet_amd/autotune.py measures the choices on the device and writes the table;
+ * nnet_ctx_create loads the table named by the environment variable NNET_MI355X_TUNING, else
+ * sph-exa-nnet_amd/data/tuning_gfx950.txt next to the library's lib/ directory when it exists
+ * (a malformed table fails the creation with NNET_E_ARG).  No counterpart in the reference,
+ * whose launch shape is a compile-time constant (parallel/CUDA/parallel_nuclear_net.cuh:51-64). */
+typedef struct nnet_tuning {
+    int ppw[3][2][2]; /* particles per workgroup, [net14, net86, net87][eos][precision] */
+    int fused;        /* see nnet_ctx_set_fused_launch */
+} nnet_tuning;
+/* Host only (no device): parse a table into `out`, starting from the defaults. */
+int nnet_tuning_read(const char* path, nnet_tuning* out);
+int nnet_ctx_load_tuning(nnet_ctx* ctx, const char* path);
+int nnet_ctx_get_tuning(nnet_ctx* ctx, nnet_tuning* out);
 
 /* Blocking variant of nnet_compute (drop-in for the reference's synchronous launch). */
 int nnet_compute_sync(nnet_ctx* ctx, const nnet_batch* batches, int nb, double hydro_dt,

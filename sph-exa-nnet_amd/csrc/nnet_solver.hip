@@ -15,6 +15,7 @@
 //              every lane (uniform control flow) with exact replay-stall detection.
 // A grid of one workgroup per particle lets the hardware dispatcher balance particles of very
 // different stiffness across the CUs.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -98,18 +99,27 @@ struct NetDev
     const uint4*   lanemap; // [lane]: register-slot mask (x, y), first non-zero id (z)
 };
 
-// Passed by value in the kernel arguments: no device allocation per call.
-struct EosDev
+// Passed by value in the kernel arguments: no device allocation per call.  The single-batch
+// kernel takes EosDev / BatchDev with the charges and abundance pointers inline; the fused
+// multi-network kernel keeps them in shared arrays and hands the solve an EosView / BatchView.
+struct EosCore
 {
     int             kind;
     int             zdim;
     double          mu;
     const HelmGrid* grid;
     const double*   cells;
-    double          Z[MAXSP];
+};
+struct EosDev : EosCore
+{
+    double Z[MAXSP];
+};
+struct EosView : EosCore
+{
+    const double* Z;
 };
 
-struct BatchDev
+struct BatchCore
 {
     size_t        n;
     const double* rho;
@@ -130,7 +140,14 @@ struct BatchDev
     uint32_t*     redo;
     unsigned int* redo_count;
     int           fp64_pass; // FP64 pass of a mixed batch: fp64_solves = newton
-    double*       Y[MAXSP]; // per-species abundance arrays (SoA)
+};
+struct BatchDev : BatchCore
+{
+    double* Y[MAXSP]; // per-species abundance arrays (SoA)
+};
+struct BatchView : BatchCore
+{
+    double* const* Y;
 };
 
 struct StepParams
@@ -183,6 +200,35 @@ __device__ __forceinline__ float bcast_lane(float x, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
 }
 
+// Barrier of the lanes that integrate one particle: the workgroup barrier when the particle has
+// the workgroup to itself; a wavefront barrier when a one-wavefront particle shares its workgroup
+// with others (the LDS instructions of one wavefront execute in program order; the fences keep
+// the compiler from moving memory accesses across the barrier).
+template<bool WAVE>
+__device__ __forceinline__ void psync()
+{
+    if constexpr (WAVE)
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    else
+        __syncthreads();
+}
+// the barrier, returning whether x is true in any lane of the particle
+template<bool WAVE>
+__device__ __forceinline__ bool psync_or(int x)
+{
+    if constexpr (WAVE)
+    {
+        psync<true>();
+        return __ballot(x) != 0;
+    }
+    else
+        return __syncthreads_or(x) != 0;
+}
+
 template<typename R>
 struct TileT;
 template<>
@@ -217,7 +263,7 @@ struct TileT<float>
     }
 };
 
-template<int NET, typename R, bool STORE_L>
+template<int NET, typename R, bool STORE_L, bool WAVE>
 struct TileLU
 {
     using C = Cfg<NET>;
@@ -378,7 +424,7 @@ struct TileLU
                 }
             }
         }
-        __syncthreads();
+        psync<WAVE>();
 
         // rank-4 trailing update, one MFMA per tile that holds trailing rows and columns
         // (A operand: row arow(l&15) of the tile, pivot l>>4; B operand: pivot l>>4, column l&15)
@@ -418,7 +464,7 @@ struct TileLU
         if constexpr (STEP + 1 < NS) publish<STEP + 1>(Mt, pan, wr, wc, lr, lc);
         __builtin_amdgcn_sched_barrier(0);
         update(false);
-        __syncthreads();
+        psync<WAVE>();
         if constexpr (STEP + 1 < NS) step<STEP + 1>(Mt, pan, sU, sL, sRinv, wr, wc, lr, lc, tid);
     }
 
@@ -428,7 +474,7 @@ struct TileLU
                                                   int lr, int lc, int tid)
     {
         publish<0>(Mt, pan, wr, wc, lr, lc);
-        __syncthreads();
+        psync<WAVE>();
         step<0>(Mt, pan, sU, sL, sRinv, wr, wc, lr, lc, tid);
     }
 
@@ -549,21 +595,26 @@ __device__ __forceinline__ int special_task(int tid)
 }
 
 // ---------------------------------------------------------------------------------------------
-// The FP64 kernel: persistent workgroups take particles from a work queue; one workgroup
-// integrates one particle over one hydro step at a time.
+// The solve: the Cfg<NET>::NT lanes of one particle group take particles from the batch's work
+// queue and integrate each over one hydro step, until the queue is empty.  sm: the group's
+// Lds<NET, MIX>::total doubles of LDS; simg: the network's assembly image, staged in LDS by the
+// caller; s_pf: two LDS words of the group; tid: the lane's index in the group.  WAVE: the group
+// is one wavefront that shares its workgroup with other groups (wavefront barriers only).
+// EosT / BatchT: the EOS and batch descriptions (EosDev / BatchDev, or views of the same fields).
 // ---------------------------------------------------------------------------------------------
-template<int NET, int EOSK, bool MIX>
-__global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
-    nnet_solve_kernel(NetDev nd, EosDev ed, BatchDev bd, StepParams sp)
+template<int NET, int EOSK, bool MIX, bool WAVE, class EosT, class BatchT>
+__device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed, const BatchT& bd,
+                                                const StepParams& sp, double* sm, const uint32_t* simg,
+                                                unsigned* s_pf, int tid)
 {
     using C  = Cfg<NET>;
     using L  = Lds<NET, MIX>;
     constexpr int D = C::D, N = C::N, NT = C::NT, WR = C::WR, TI = C::TT / C::WR, NA = (N + 63) / 64;
     constexpr int Dr = (NET == 14) ? 14 : 86; // species of the rate functor's BE/screening loops
+    static_assert(!WAVE || NT == 64, "wavefront barriers: one-wavefront particle groups only");
 
-    __shared__ __align__(16) double sm[L::total];
-    __shared__ uint32_t simg[C::IMG];
-    __shared__ unsigned s_p, s_flag;
+    unsigned& s_p    = s_pf[0];
+    unsigned& s_flag = s_pf[1];
     double* sBE   = sm + L::oBE;
     double* sMu   = sm + L::oMu;
     double* sEff  = sm + L::oEff;
@@ -589,12 +640,8 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
     double* sPan  = sm + L::oPan;
     double* sRow0 = sm + L::oRow0;
 
-    const int tid = threadIdx.x;
     const int lr = (tid & 63) >> 4, lc = tid & 15, wr = (tid >> 6) / WR, wc = (tid >> 6) % WR;
 
-    // stage the network's assembly image once per workgroup
-    for (int w = tid; w < nd.img_words; w += NT)
-        simg[w] = nd.img[w];
     const uint32_t* rx     = simg + nd.oRx;
     const uint16_t* ent16  = reinterpret_cast<const uint16_t*>(simg) + nd.oEnt16;
     const uint16_t* spp16  = reinterpret_cast<const uint16_t*>(simg) + nd.oSpPtr16;
@@ -608,9 +655,9 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
     const size_t q0 = bd.qrange ? (size_t)bd.qrange[0] : 0, q1 = bd.qrange ? (size_t)bd.qrange[1] : bd.n;
     for (;;)
     {
-        __syncthreads();
+        psync<WAVE>();
         if (tid == 0) s_p = atomicAdd(bd.queue, 1u);
-        __syncthreads();
+        psync<WAVE>();
         if (q0 + s_p >= q1) break;
         const size_t p = bd.order ? bd.order[q0 + s_p] : s_p;
 
@@ -680,7 +727,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                 const double yb = (i <= 1) ? sY[s] : sYb[s];
                 sYb[s]          = (1 - k::theta) * sY[s] + k::theta * yb;
             }
-            __syncthreads();
+            psync<WAVE>();
 
             // ---- EOS (one lane) and rates, phase 1 (all lanes) ----
             if constexpr (EOSK != NNET_EOS_IDEAL)
@@ -835,7 +882,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                     electron_interpolate(nd.elogT, nd.elogR, nd.etab, Tg, sYb[86] * rho_t, sEv);
                 if (NET != 14 && tid == 225) sEff[3] = sDeff[3] = sL[3] = sDl[3] = 0; // unused slots
             }
-            __syncthreads();
+            psync<WAVE>();
 
             // ---- rates, phase 2: swap, screening, reaction-list order ----
             {
@@ -886,7 +933,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                     sDrt[310]            = rhoElec == 0 ? 0 : sEv[7] / rhoElec;
                 }
             }
-            __syncthreads();
+            psync<WAVE>();
 
             // ---- per-reaction mass-action factors (nuclear_net.hpp:355-371, 404-426) ----
             for (int r = tid; r < nd.nR; r += NT)
@@ -922,7 +969,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                         sTr[r * MAXQ + (qa - q0)] = tr_;
                     }
             }
-            __syncthreads();
+            psync<WAVE>();
 
             // ---- Jacobian non-zeros (firstOrderDYFromReactions) and species sums
             //      (derivativesFromReactions), each summed in the reference's order ----
@@ -1017,7 +1064,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             }
             // this lane's register-slot mask and first non-zero id (for the matrix fill)
             const uint4 lmap = nd.lanemap[tid];
-            __syncthreads();
+            psync<WAVE>();
             // RHS correction sum_j J[a][j] (Yg[j] - Y[j]) over the row's non-zeros, j ascending
             if (tid < D)
             {
@@ -1053,7 +1100,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                 }
                 sCorr[tid] = corr;
             }
-            __syncthreads();
+            psync<WAVE>();
 
             // ---- Newton matrix into registers (nuclear_net.hpp:551-608) ----
             double cvv, dpdTv;
@@ -1125,9 +1172,9 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
             {
                 dx4 Mt[TI][TI];
                 fill(Mt);
-                __syncthreads(); // the packed rows below alias the assembly arrays
+                psync<WAVE>(); // the packed rows below alias the assembly arrays
                 // ---- LU (eigen.hpp:179-206), four pivots per block ----
-                TileLU<NET, double, false>::factor(Mt, sPan, sU, nullptr, nullptr, wr, wc, lr, lc, tid);
+                TileLU<NET, double, false, WAVE>::factor(Mt, sPan, sU, nullptr, nullptr, wr, wc, lr, lc, tid);
                 // ---- back substitution (eigen.hpp:208-221) ----
                 if (tid < 64)
                 {
@@ -1136,15 +1183,15 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                     for (int a = 0; a < NA; ++a)
                     {
                         const int row = tid + 64 * a;
-                        const double x = sU[row < N ? TileLU<NET, double, false>::urow(row) + N : 0];
+                        const double x = sU[row < N ? TileLU<NET, double, false, WAVE>::urow(row) + N : 0];
                         acc[a]         = row < N ? x : 0.;
                     }
-                    TileLU<NET, double, false>::back(sU, acc, tid);
+                    TileLU<NET, double, false, WAVE>::back(sU, acc, tid);
 #pragma unroll
                     for (int a = 0; a < NA; ++a)
                         if (tid + 64 * a < N) sX[tid + 64 * a] = acc[a];
                 }
-                __syncthreads();
+                psync<WAVE>();
             };
 
             // the FP64 solve has a single (inlined) call site, so that every LDS access of the
@@ -1156,7 +1203,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                 // ---- mixed precision: FP32 tile LU + FP64 iterative refinement; a particle
                 //      with an iteration whose refinement does not converge is abandoned here
                 //      and integrated again, from the start of the step, by the FP64 pass ----
-                using LU32 = TileLU<NET, float, true>;
+                using LU32 = TileLU<NET, float, true, WAVE>;
                 float* sU32   = reinterpret_cast<float*>(sm);
                 float* sL32   = sU32 + L::NU;
                 float* sRi32  = sL32 + L::NL;
@@ -1164,7 +1211,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                 {
                     fx4 Mt[TI][TI];
                     fill(Mt);
-                    __syncthreads(); // FP32 factors alias the head arrays only
+                    psync<WAVE>(); // FP32 factors alias the head arrays only
                     LU32::factor(Mt, sPan32, sU32, sL32, sRi32, wr, wc, lr, lc, tid);
                 }
                 if (tid < 64)
@@ -1182,7 +1229,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                     for (int a = 0; a < NA; ++a)
                         if (tid + 64 * a < N) sX[tid + 64 * a] = (double)acc[a];
                 }
-                __syncthreads();
+                psync<WAVE>();
                 double* sRes = sPan; // the panel buffers are free after the LU
                 bool    conv = false;
                 for (int sweep = 0; sweep < k::mixed_sweeps(NET) && !conv; ++sweep)
@@ -1214,7 +1261,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                         }
                         sRes[ri] = r;
                     }
-                    __syncthreads();
+                    psync<WAVE>();
                     // correction d = (LU)^-1 r in FP32; converged when every |d_i| is below
                     // mixedTol of its magnitude scale (|x_0| + T for the temperature, |x_i| + Y_i
                     // for the abundances)
@@ -1246,7 +1293,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                         const bool all = __all(ok);
                         if (tid == 0) s_flag = all ? 1u : 0u;
                     }
-                    __syncthreads();
+                    psync<WAVE>();
                     conv = s_flag != 0;
                 }
                 abandon = !conv;
@@ -1270,7 +1317,7 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                 yb_new  = sY[tid] + sX[tid + 1];
                 nanflag = isnan(yb_new) ? 1 : 0;
             }
-            const bool has_nan = __syncthreads_or(nanflag) || isnan(T_new);
+            const bool has_nan = psync_or<WAVE>(nanflag) || isnan(T_new);
             if (has_nan || T_new < 0)
             {
                 u *= k::nanDtStep;
@@ -1334,9 +1381,9 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
                 status = NNET_STATUS_MAX_ITER;
                 break;
             }
-            __syncthreads();
+            psync<WAVE>();
         }
-        __syncthreads();
+        psync<WAVE>();
         if constexpr (MIX)
             if (abandon)
             {
@@ -1357,21 +1404,136 @@ __global__ void __launch_bounds__(Cfg<NET>::NT, Cfg<NET>::WPE)
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// The solve kernel of one batch: persistent workgroups of PPW particle groups take particles
+// from the batch's work queue.  PPW > 1 (one-wavefront networks only): the groups of a workgroup
+// share the staged assembly image and otherwise run independently (wavefront barriers).
+// ---------------------------------------------------------------------------------------------
+template<int NET, int EOSK, bool MIX, int PPW>
+__global__ void __launch_bounds__(Cfg<NET>::NT * PPW, Cfg<NET>::WPE)
+    nnet_solve_kernel(NetDev nd, EosDev ed, BatchDev bd, StepParams sp)
+{
+    using C = Cfg<NET>;
+    using L = Lds<NET, MIX>;
+    static_assert(PPW == 1 || C::NT == 64, "several particles per workgroup: one-wavefront networks only");
+    __shared__ __align__(16) double sm[PPW * L::total];
+    __shared__ uint32_t simg[C::IMG];
+    __shared__ unsigned s_pf[2 * PPW];
+    const int group = PPW == 1 ? 0 : (int)threadIdx.x / C::NT;
+    const int tid   = PPW == 1 ? (int)threadIdx.x : (int)threadIdx.x % C::NT;
+    // stage the network's assembly image once per workgroup
+    for (int w = threadIdx.x; w < nd.img_words; w += C::NT * PPW)
+        simg[w] = nd.img[w];
+    __syncthreads();
+    solve_particles<NET, EOSK, MIX, (PPW > 1)>(nd, ed, bd, sp, sm + group * L::total, simg, s_pf + 2 * group, tid);
+}
+
+// ---------------------------------------------------------------------------------------------
+// One launch for the batches of several networks with one EOS and one precision.  Each
+// workgroup drains the batches' work queues phase by phase (the host orders the phases by cost:
+// net87 and net86 before net14), staging each phase's assembly image; net86 / net87 particles
+// take the whole workgroup, net14 particles one wavefront each (four per workgroup).  The
+// charges and abundance pointers of all phases share two arrays of FUSE_SPECIES entries, which
+// keeps the argument block (passed by value, so a captured HIP graph holds its own copy) under
+// 4 KiB.
+// ---------------------------------------------------------------------------------------------
+constexpr int FUSE_PHASES = 4, FUSE_SPECIES = 192, FUSE_NT = 256;
+struct PhaseDev
+{
+    const NetDev* nd;   // device copy of the network's NetDev
+    int           kind; // 14, 86 or 87
+    int           zoff; // first charge of the phase in FusedDev::Z
+    int           yoff; // first abundance pointer of the phase in FusedDev::Y
+    EosCore       eos;
+    BatchCore     b;
+};
+struct FusedDev
+{
+    int      nphase;
+    PhaseDev ph[FUSE_PHASES];
+    double   Z[FUSE_SPECIES];
+    double*  Y[FUSE_SPECIES];
+};
+static_assert(sizeof(FusedDev) + sizeof(StepParams) <= 4096, "fused kernel arguments within 4 KiB");
+
+template<bool MIX>
+struct FusedLds
+{
+    static constexpr int m2(int a, int b) { return a > b ? a : b; }
+    static constexpr int total =
+        m2(m2(Lds<86, MIX>::total, Lds<87, MIX>::total), (FUSE_NT / 64) * Lds<14, MIX>::total);
+    static constexpr int img = m2(m2(Cfg<86>::IMG, Cfg<87>::IMG), Cfg<14>::IMG);
+};
+static_assert(Cfg<86>::NT == FUSE_NT && Cfg<87>::NT == FUSE_NT && Cfg<14>::NT == 64, "fused workgroup shape");
+
+template<int EOSK, bool MIX>
+__global__ void __launch_bounds__(FUSE_NT, 2) nnet_solve_fused(FusedDev fd, StepParams sp)
+{
+    using F = FusedLds<MIX>;
+    __shared__ __align__(16) double sm[F::total];
+    __shared__ uint32_t simg[F::img];
+    __shared__ unsigned s_pf[2 * (FUSE_NT / 64)];
+    for (int ph = 0; ph < fd.nphase; ++ph)
+    {
+        const PhaseDev& P  = fd.ph[ph];
+        const NetDev&   nd = *P.nd;
+        EosView         ev;
+        static_cast<EosCore&>(ev) = P.eos;
+        ev.Z                      = fd.Z + P.zoff;
+        BatchView bv;
+        static_cast<BatchCore&>(bv) = P.b;
+        bv.Y                        = fd.Y + P.yoff;
+        __syncthreads(); // every group is done with the previous phase's image and LDS
+        for (int w = threadIdx.x; w < nd.img_words; w += FUSE_NT)
+            simg[w] = nd.img[w];
+        __syncthreads();
+        if (P.kind == 87) solve_particles<87, EOSK, MIX, false>(nd, ev, bv, sp, sm, simg, s_pf, threadIdx.x);
+        else if (P.kind == 86)
+            solve_particles<86, EOSK, MIX, false>(nd, ev, bv, sp, sm, simg, s_pf, threadIdx.x);
+        else
+        {
+            const int g = threadIdx.x >> 6;
+            solve_particles<14, EOSK, MIX, true>(nd, ev, bv, sp, sm + g * Lds<14, MIX>::total, simg, s_pf + 2 * g,
+                                                 threadIdx.x & 63);
+        }
+    }
+}
+
 // The solve kernels are compiled in translation units of their own, one per network (this file
 // built with NNET_SOLVE_NET=<network>, which keeps only the code above), so the library's
 // objects build in parallel; the host part below launches them through these declarations.
-#define NNET_SOLVE_INSTANCES(SPEC, NET)                                                                      \
-    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_IDEAL, false>(NetDev, EosDev, BatchDev, StepParams);     \
-    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_IDEAL, true>(NetDev, EosDev, BatchDev, StepParams);      \
-    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_HELMHOLTZ, false>(NetDev, EosDev, BatchDev, StepParams); \
-    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_HELMHOLTZ, true>(NetDev, EosDev, BatchDev, StepParams);
+#define NNET_SOLVE_INSTANCE(SPEC, NET, PPW)                                                                           \
+    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_IDEAL, false, PPW>(NetDev, EosDev, BatchDev,         \
+                                                                                     StepParams);                     \
+    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_IDEAL, true, PPW>(NetDev, EosDev, BatchDev,          \
+                                                                                    StepParams);                      \
+    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_HELMHOLTZ, false, PPW>(NetDev, EosDev, BatchDev,     \
+                                                                                         StepParams);                 \
+    SPEC template __global__ void nnet_solve_kernel<NET, NNET_EOS_HELMHOLTZ, true, PPW>(NetDev, EosDev, BatchDev,      \
+                                                                                        StepParams);
+#define NNET_SOLVE_INSTANCES(SPEC, NET)                                                                               \
+    NNET_SOLVE_INSTANCE(SPEC, NET, 1)                                                                                 \
+    NNET_SOLVE_INSTANCE(SPEC, NET, 2)                                                                                 \
+    NNET_SOLVE_INSTANCE(SPEC, NET, 4)
+// NNET_SOLVE_NET = 1000 + 2 eos + mixed: the fused kernel of that EOS and precision
+#define NNET_FUSED_INSTANCE(SPEC, EOSK, MIX) SPEC template __global__ void nnet_solve_fused<EOSK, MIX>(FusedDev, StepParams);
 #ifdef NNET_SOLVE_NET
-NNET_SOLVE_INSTANCES(, NNET_SOLVE_NET)
+#if NNET_SOLVE_NET == 14
+NNET_SOLVE_INSTANCES(, 14)
+#elif NNET_SOLVE_NET >= 1000
+NNET_FUSED_INSTANCE(, (NNET_SOLVE_NET - 1000) / 2, (NNET_SOLVE_NET - 1000) % 2 == 1)
+#else
+NNET_SOLVE_INSTANCE(, NNET_SOLVE_NET, 1)
+#endif
 } // namespace nnet_amd
 #else
 NNET_SOLVE_INSTANCES(extern, 14)
-NNET_SOLVE_INSTANCES(extern, 86)
-NNET_SOLVE_INSTANCES(extern, 87)
+NNET_SOLVE_INSTANCE(extern, 86, 1)
+NNET_SOLVE_INSTANCE(extern, 87, 1)
+NNET_FUSED_INSTANCE(extern, NNET_EOS_IDEAL, false)
+NNET_FUSED_INSTANCE(extern, NNET_EOS_IDEAL, true)
+NNET_FUSED_INSTANCE(extern, NNET_EOS_HELMHOLTZ, false)
+NNET_FUSED_INSTANCE(extern, NNET_EOS_HELMHOLTZ, true)
 
 // ---------------------------------------------------------------------------------------------
 // Post-step EOS refresh (computeHelmholtzImpl, parallel_nuclear_net_impl.cpp:116-141): one lane
@@ -1518,11 +1680,25 @@ __global__ void __launch_bounds__(256) nnet_order_scatter(const uint32_t* __rest
 // =============================================================================================
 using namespace nnet_amd;
 
+// Launch parameters of the solve kernels per network, EOS and precision: the values of the
+// autotuner's table (sph-exa-nnet_amd/autotune.py; nnet_ctx_load_tuning) or of
+// nnet_ctx_set_particles_per_workgroup.
+struct Tuning
+{
+    // particle groups per workgroup of net14, [eos][mixed]; net86 / net87 always run one
+    int ppw14[2][2] = {{1, 1}, {1, 1}};
+    int ppw(int net, int eos, bool mixed) const { return net == 14 ? ppw14[eos ? 1 : 0][mixed ? 1 : 0] : 1; }
+    // calls with batches of several networks: one fused launch (1) or one launch per batch on side
+    // streams (0)
+    int fused = 1;
+};
+
 struct nnet_ctx
 {
     int         device = 0;
     std::string err;
     long        max_iter = 10000;
+    Tuning      tuning;
     // Helmholtz
     HelmGrid* d_grid  = nullptr;
     double*   d_cells  = nullptr;
@@ -1564,7 +1740,8 @@ struct nnet_network
     int              kind = 0, d = 0, nR = 0;
     std::vector<int> rb, pb, ids, nums;
     NetDev           dev{};
-    void*            blob = nullptr;
+    const NetDev*    d_dev = nullptr; // device copy of `dev` (inside `blob`), for the fused launch
+    void*            blob  = nullptr;
 };
 
 static int fail(nnet_ctx* c, int code, const std::string& msg)
@@ -1639,7 +1816,7 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
            oScl = bl.add(scl), oPe = bl.add(pe), oPl = bl.add(pl), oElT = bl.add(NNET_ELEC_LOGT, NNET_ELEC_NT),
            oElR = bl.add(NNET_ELEC_LOGRHO, NNET_ELEC_NR),
            oElTab = bl.add(NNET_ELEC_RATE, (size_t)NNET_ELEC_NT * NNET_ELEC_NR * NNET_ELEC_NC), oImg = bl.add(img),
-           oLane = bl.add(lanemap);
+           oLane = bl.add(lanemap), oSelf = bl.add(std::vector<char>(sizeof(NetDev)));
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipMalloc(&nw->blob, bl.b.size()));
     HIP_TRY(ctx, hipMemcpy(nw->blob, bl.b.data(), bl.b.size(), hipMemcpyHostToDevice));
@@ -1656,6 +1833,8 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
     n.img = (uint32_t*)(B + oImg), n.img_words = (int)img.size(), n.oRx = im.oRx, n.oEnt16 = im.oEnt16;
     n.oSpPtr16 = im.oSpPtr16, n.oSpEnt = im.oSpEnt, n.oNzPtr16 = im.oNzPtr16, n.oNzEnt = im.oNzEnt;
     n.oRowPtr16 = im.oRowPtr16, n.oRowEnt = im.oRowEnt, n.oJord16 = im.oJord16, n.lanemap = (const uint4*)(B + oLane);
+    HIP_TRY(ctx, hipMemcpy(B + oSelf, &n, sizeof(NetDev), hipMemcpyHostToDevice));
+    nw->d_dev = (const NetDev*)(B + oSelf);
     return NNET_OK;
 }
 
@@ -1712,6 +1891,81 @@ static int scratch_ptr_of(int device, uint32_t** out)
     g_scratch_ptr[device] = static_cast<uint32_t*>(p);
     *out                  = static_cast<uint32_t*>(p);
     return NNET_OK;
+}
+
+// The autotuner's table (format in nnet_mi355x.h): settings applied over `t`, which is left
+// unchanged when the table cannot be read or has a malformed line.
+static int tuning_parse(const char* path, Tuning& t, std::string& err)
+{
+    FILE* f = path ? std::fopen(path, "r") : nullptr;
+    if (!f)
+    {
+        err = std::string("cannot open the tuning table ") + (path ? path : "(null)");
+        return NNET_E_ARG;
+    }
+    Tuning r  = t;
+    char   line[512];
+    int    ln = 0, rc = NNET_OK;
+    while (rc == NNET_OK && std::fgets(line, sizeof line, f))
+    {
+        ++ln;
+        if (char* h = std::strchr(line, '#')) *h = 0;
+        char key[32], extra[8];
+        int  kind, eos, prec, v;
+        if (std::sscanf(line, " %31s", key) != 1) continue;
+        bool ok = false;
+        if (!std::strcmp(key, "ppw") && std::sscanf(line, " ppw %d %d %d %d %7s", &kind, &eos, &prec, &v, extra) == 4 &&
+            (kind == 14 || kind == 86 || kind == 87) && (eos == 0 || eos == 1) && (prec == 0 || prec == 1) &&
+            (kind == 14 ? (v == 1 || v == 2 || v == 4) : v == 1))
+        {
+            if (kind == 14) r.ppw14[eos][prec] = v;
+            ok = true;
+        }
+        else if (!std::strcmp(key, "fused") && std::sscanf(line, " fused %d %7s", &v, extra) == 1 && (v == 0 || v == 1))
+        {
+            r.fused = v;
+            ok      = true;
+        }
+        if (!ok)
+        {
+            err = std::string("tuning table ") + path + ", line " + std::to_string(ln) +
+                  ": expected 'ppw <14|86|87> <eos 0|1> <precision 0|1> <1|2|4 (net14), 1>' or 'fused <0|1>'";
+            rc = NNET_E_ARG;
+        }
+    }
+    std::fclose(f);
+    if (rc == NNET_OK) t = r;
+    return rc;
+}
+
+// The table nnet_ctx_create loads: $NNET_MI355X_TUNING (empty: none), else
+// data/tuning_gfx950.txt next to the directory of this library when it exists.
+static std::string default_tuning_path()
+{
+    if (const char* e = std::getenv("NNET_MI355X_TUNING")) return e;
+    Dl_info info{};
+    if (dladdr(reinterpret_cast<void*>(&default_tuning_path), &info) && info.dli_fname)
+    {
+        const std::string lib   = info.dli_fname;
+        const size_t      slash = lib.rfind('/');
+        const std::string p = (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) +
+                              "/../data/tuning_gfx950.txt";
+        if (FILE* f = std::fopen(p.c_str(), "r"))
+        {
+            std::fclose(f);
+            return p;
+        }
+    }
+    return "";
+}
+
+static void tuning_export(const Tuning& t, nnet_tuning* out)
+{
+    for (int k = 0; k < 3; ++k)
+        for (int e = 0; e < 2; ++e)
+            for (int p = 0; p < 2; ++p)
+                out->ppw[k][e][p] = k == 0 ? t.ppw14[e][p] : 1;
+    out->fused = t.fused;
 }
 
 // Argument checks of nnet_compute; nothing is allocated or enqueued.  n_live: non-empty batches.
@@ -1771,9 +2025,153 @@ static int check_capture(nnet_ctx* c, const nnet_batch* batches, int nb, int n_l
 }
 
 // Enqueues the kernels of one nnet_compute call on `st` with the given buffers.
+// Ordered dispatch of batch bi: longest first by the cost hint; with a routing hint (a mixed
+// batch), the particles with a nonzero fp64 hint form the prefix [0, nH) of the order, the rest
+// [nH, n).  Returns the order (nullptr: index order, no hint) and points `range` at the device
+// bounds {0, nH, n}.
+static const uint32_t* enqueue_order(const nnet_batch& b, int bi, const uint32_t* route, const CallBufs& bufs,
+                                     size_t& order_off, hipStream_t s, const unsigned int*& range)
+{
+    range = nullptr;
+    if (!b.cost_hint && !route) return nullptr;
+    unsigned int* bins  = bufs.bins + (size_t)BIN_STRIDE * bi;
+    uint32_t*     order = bufs.order + order_off;
+    order_off += b.n;
+    const unsigned blocks = (unsigned)std::min<size_t>((b.n + 255) / 256, 2048);
+    hipLaunchKernelGGL(nnet_queue_reset, dim3((BIN_STRIDE + 255) / 256), dim3(256), 0, s, bins, BIN_STRIDE);
+    hipLaunchKernelGGL(nnet_order_count, dim3(blocks), dim3(256), 0, s, b.cost_hint, route, b.n, bins);
+    hipLaunchKernelGGL(nnet_order_offsets, dim3(1), dim3(64), 0, s, bins);
+    hipLaunchKernelGGL(nnet_order_scatter, dim3(blocks), dim3(256), 0, s, b.cost_hint, route, b.n, bins, order);
+    range = bins + 2 * NKEY; // {0, nH, n}
+    return order;
+}
+
+// Can the call's batches run as one fused launch?  All live batches share the EOS and the
+// precision, there are at most FUSE_PHASES of them, and their abundance pointers and charges fit
+// the FUSE_SPECIES-entry arrays of the launch arguments.
+static bool fusable(const nnet_batch* batches, int nb)
+{
+    int live = 0, ys = 0, zs = 0, eos = -1, prec = -1;
+    for (int bi = 0; bi < nb; ++bi)
+    {
+        const nnet_batch& b = batches[bi];
+        if (b.n == 0) continue;
+        if (eos >= 0 && (b.eos != eos || b.precision != prec)) return false;
+        eos = b.eos, prec = b.precision;
+        ++live;
+        ys += b.network->d;
+        zs += b.eos == NNET_EOS_HELMHOLTZ ? b.z_dim : 0;
+    }
+    return live >= 2 && live <= FUSE_PHASES && ys <= FUSE_SPECIES && zs <= FUSE_SPECIES;
+}
+
+// The kernels of a fused call on `st`: per batch the queue reset and the cost ordering, then one
+// nnet_solve_fused launch over all batches (mixed: then one FP64 launch over the redo lists).
+static int enqueue_fused(nnet_ctx* c, const nnet_batch* batches, int nb, double hydro_dt, double previous_dt,
+                         int use_drhodt, hipStream_t st, const CallBufs& bufs)
+{
+    StepParams sp{hydro_dt, previous_dt, use_drhodt, c->max_iter};
+    StepParams redo_sp = sp;
+    if (sp.use_drhodt == NNET_DRHODT_CPU) redo_sp.use_drhodt = NNET_DRHODT_OFF; // rho_m1 already updated
+    // phases in cost order: net87, net86, net14; larger batches first within a network
+    std::vector<int> phase;
+    for (int bi = 0; bi < nb; ++bi)
+        if (batches[bi].n > 0) phase.push_back(bi);
+    auto rank = [&](int bi) { return batches[bi].network->kind == 87 ? 0 : batches[bi].network->kind == 86 ? 1 : 2; };
+    std::stable_sort(phase.begin(), phase.end(), [&](int x, int y)
+                     { return rank(x) != rank(y) ? rank(x) < rank(y) : batches[x].n > batches[y].n; });
+    const bool mix = batches[phase[0]].precision == NNET_PRECISION_MIXED;
+    const int  eos = batches[phase[0]].eos;
+    FusedDev   fd{}, fr{};
+    size_t     order_off = 0, redo_off = 0, work = 0;
+    int        zoff = 0, yoff = 0;
+    fd.nphase = fr.nphase = (int)phase.size();
+    for (int k = 0; k < (int)phase.size(); ++k)
+    {
+        const int         bi  = phase[k];
+        const nnet_batch& b   = batches[bi];
+        const int         d   = b.network->d;
+        unsigned int*     ctr = bufs.queue + (size_t)nnet_ctx::NCTR * bi;
+        hipLaunchKernelGGL(nnet_queue_reset, dim3(1), dim3(64), 0, st, ctr, nnet_ctx::NCTR);
+        PhaseDev& P = fd.ph[k];
+        P.nd        = b.network->d_dev;
+        P.kind      = b.network->kind;
+        P.eos.kind  = b.eos;
+        P.eos.mu    = b.mu;
+        P.zoff      = zoff;
+        if (b.eos == NNET_EOS_HELMHOLTZ)
+        {
+            for (int z = 0; z < b.z_dim; ++z)
+                fd.Z[zoff + z] = b.Z[z];
+            zoff += b.z_dim;
+            P.eos.zdim  = b.z_dim;
+            P.eos.grid  = c->d_grid;
+            P.eos.cells = c->d_cells;
+        }
+        P.yoff = yoff;
+        for (int s = 0; s < d; ++s)
+            fd.Y[yoff + s] = b.Y[s];
+        yoff += d;
+        BatchCore& bc = P.b;
+        bc.n = b.n, bc.rho = b.rho, bc.rho_m1 = b.rho_m1, bc.temp = b.temp, bc.dt = b.dt;
+        bc.status = b.status, bc.substeps = b.substeps, bc.newton = b.newton, bc.fp64_solves = b.fp64_solves;
+        bc.queue = ctr;
+        if (mix)
+        {
+            bc.redo       = bufs.redo + redo_off;
+            bc.redo_count = ctr + 4;
+            redo_off += b.n;
+        }
+        const unsigned int* range = nullptr;
+        bc.order                  = enqueue_order(b, bi, nullptr, bufs, order_off, st, range);
+        // the FP64 pass over the redo list: its own queue, the list as the order, {0, count}
+        fr.ph[k]             = P;
+        fr.ph[k].b.queue     = ctr + 2;
+        fr.ph[k].b.order     = bc.redo;
+        fr.ph[k].b.qrange    = ctr + 3;
+        fr.ph[k].b.fp64_pass = 1;
+        work += P.kind == 14 ? (b.n + 3) / 4 : b.n;
+    }
+    std::memcpy(fr.Z, fd.Z, sizeof(fd.Z));
+    std::memcpy(fr.Y, fd.Y, sizeof(fd.Y));
+    auto go = [&](auto kern, const FusedDev& a, const StepParams& s)
+    {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, FUSE_NT, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        const size_t wgs = std::min<size_t>(work, (size_t)per_cu * c->n_cu);
+        hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(FUSE_NT), 0, st, a, s);
+    };
+    if (eos == NNET_EOS_IDEAL)
+    {
+        if (mix)
+        {
+            go(nnet_solve_fused<NNET_EOS_IDEAL, true>, fd, sp);
+            go(nnet_solve_fused<NNET_EOS_IDEAL, false>, fr, redo_sp);
+        }
+        else
+            go(nnet_solve_fused<NNET_EOS_IDEAL, false>, fd, sp);
+    }
+    else
+    {
+        if (mix)
+        {
+            go(nnet_solve_fused<NNET_EOS_HELMHOLTZ, true>, fd, sp);
+            go(nnet_solve_fused<NNET_EOS_HELMHOLTZ, false>, fr, redo_sp);
+        }
+        else
+            go(nnet_solve_fused<NNET_EOS_HELMHOLTZ, false>, fd, sp);
+    }
+    HIP_TRY(c, hipGetLastError());
+    return NNET_OK;
+}
+
 static int enqueue_compute(nnet_ctx* c, const nnet_batch* batches, int nb, double hydro_dt, double previous_dt,
                            int use_drhodt, hipStream_t st, int n_live, const CallBufs& bufs)
 {
+    // batches of several networks: one fused launch when the context's tuning selects it
+    if (n_live > 1 && c->tuning.fused && fusable(batches, nb))
+        return enqueue_fused(c, batches, nb, hydro_dt, previous_dt, use_drhodt, st, bufs);
     StepParams sp{hydro_dt, previous_dt, use_drhodt, c->max_iter};
     const bool fork = n_live > 1;
     // the fp64 routing hint forks an auxiliary stream per batch; it is honoured in single-batch
@@ -1820,26 +2218,9 @@ static int enqueue_compute(nnet_ctx* c, const nnet_batch* batches, int nb, doubl
             bd.redo_count = ctr + 4;
             redo_off += b.n;
         }
-        // ordered dispatch: longest first by the cost hint; in a mixed batch, the particles with
-        // a nonzero fp64 hint form the prefix [0, nH) of the order, the rest [nH, n)
         const uint32_t*     route = (mix && allow_route) ? b.fp64_hint : nullptr;
         const unsigned int* range = nullptr;
-        bd.order                  = nullptr;
-        if (b.cost_hint || route)
-        {
-            unsigned int* bins  = bufs.bins + (size_t)BIN_STRIDE * bi;
-            uint32_t*     order = bufs.order + order_off;
-            order_off += b.n;
-            const unsigned blocks = (unsigned)std::min<size_t>((b.n + 255) / 256, 2048);
-            hipLaunchKernelGGL(nnet_queue_reset, dim3((BIN_STRIDE + 255) / 256), dim3(256), 0, bst, bins,
-                               BIN_STRIDE);
-            hipLaunchKernelGGL(nnet_order_count, dim3(blocks), dim3(256), 0, bst, b.cost_hint, route, b.n, bins);
-            hipLaunchKernelGGL(nnet_order_offsets, dim3(1), dim3(64), 0, bst, bins);
-            hipLaunchKernelGGL(nnet_order_scatter, dim3(blocks), dim3(256), 0, bst, b.cost_hint, route, b.n, bins,
-                               order);
-            bd.order = order;
-            range    = bins + 2 * NKEY; // {0, nH, n}
-        }
+        bd.order                  = enqueue_order(b, bi, route, bufs, order_off, bst, range);
         const int         kind = b.network->kind;
         const int         ak   = fork ? (live - 1) % nnet_ctx::NSIDE : 0;
         const hipStream_t ast  = c->aux[ak];
@@ -1848,14 +2229,36 @@ static int enqueue_compute(nnet_ctx* c, const nnet_batch* batches, int nb, doubl
         {
             if (e != hipSuccess && herr == hipSuccess) herr = e;
         };
-        auto              launch = [&](auto kern, int nt, const BatchDev& bdv, const StepParams& spv, hipStream_t s)
+        // the solve kernel of the batch's network with the context's particle groups per workgroup
+        // (net14: 1, 2 or 4): as many workgroups as are resident at once, at most one group per
+        // particle
+        auto launch = [&](auto netc, auto eosc, auto mixc, const BatchDev& bdv, const StepParams& spv, hipStream_t s)
         {
-            int per_cu = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, 0) != hipSuccess || per_cu < 1)
-                per_cu = 1;
-            const size_t wgs = std::min<size_t>(b.n, (size_t)per_cu * c->n_cu);
-            hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(nt), 0, s, nd, ed, bdv, spv);
+            constexpr int  NV = decltype(netc)::value, EV = decltype(eosc)::value;
+            constexpr bool MV = decltype(mixc)::value;
+            auto           go = [&](auto kern, int ppw)
+            {
+                const int nt     = Cfg<NV>::NT * ppw;
+                int       per_cu = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, 0) != hipSuccess || per_cu < 1)
+                    per_cu = 1;
+                const size_t wgs = std::min<size_t>((b.n + ppw - 1) / ppw, (size_t)per_cu * c->n_cu);
+                hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(nt), 0, s, nd, ed, bdv, spv);
+            };
+            const int ppw = c->tuning.ppw(NV, EV, MV);
+            if constexpr (NV == 14)
+            {
+                if (ppw == 4) go(nnet_solve_kernel<NV, EV, MV, 4>, 4);
+                else if (ppw == 2)
+                    go(nnet_solve_kernel<NV, EV, MV, 2>, 2);
+                else
+                    go(nnet_solve_kernel<NV, EV, MV, 1>, 1);
+            }
+            else
+                go(nnet_solve_kernel<NV, EV, MV, 1>, 1);
         };
+        using Fp64  = std::false_type;
+        using Mixed = std::true_type;
         // mixed batch: the routed particles [0, nH) go to an FP64 pass on the aux stream while
         // the mixed pass takes [nH, n); the particles the mixed pass gives up go to a last FP64
         // pass over its redo list (rho_m1 already updated by the mixed pass)
@@ -1874,20 +2277,19 @@ static int enqueue_compute(nnet_ctx* c, const nnet_batch* batches, int nb, doubl
         if (sp.use_drhodt == NNET_DRHODT_CPU) redo_sp.use_drhodt = NNET_DRHODT_OFF; // rho_m1 already updated
         auto dispatch       = [&](auto netc, auto eosc)
         {
-            constexpr int NV = decltype(netc)::value, EV = decltype(eosc)::value;
             if (!mix)
             {
-                launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, bd, sp, bst);
+                launch(netc, eosc, Fp64{}, bd, sp, bst);
                 return;
             }
             if (route)
             {
                 chk(hipEventRecord(c->aux_fork[ak], bst));
                 chk(hipStreamWaitEvent(ast, c->aux_fork[ak], 0));
-                launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, routed_bd, sp, ast);
+                launch(netc, eosc, Fp64{}, routed_bd, sp, ast);
             }
-            launch(nnet_solve_kernel<NV, EV, true>, Cfg<NV>::NT, mixed_bd, sp, bst);
-            launch(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT, redo_bd, redo_sp, bst);
+            launch(netc, eosc, Mixed{}, mixed_bd, sp, bst);
+            launch(netc, eosc, Fp64{}, redo_bd, redo_sp, bst);
             if (route)
             {
                 chk(hipEventRecord(c->aux_join[ak], ast));
@@ -1954,7 +2356,37 @@ extern "C"
             return NNET_E_HIP;
         }
         *out = c;
+        const std::string tp = default_tuning_path();
+        if (!tp.empty())
+        {
+            const int rc = tuning_parse(tp.c_str(), c->tuning, c->err);
+            if (rc != NNET_OK) return rc;
+        }
         if (helm_table_path) return nnet_ctx_load_helmholtz(c, helm_table_path);
+        return NNET_OK;
+    }
+
+    int nnet_tuning_read(const char* path, nnet_tuning* out)
+    {
+        if (!path || !out) return NNET_E_ARG;
+        Tuning      t;
+        std::string err;
+        const int   rc = tuning_parse(path, t, err);
+        if (rc != NNET_OK) return rc;
+        tuning_export(t, out);
+        return NNET_OK;
+    }
+
+    int nnet_ctx_load_tuning(nnet_ctx* c, const char* path)
+    {
+        if (!c || !path) return NNET_E_ARG;
+        return tuning_parse(path, c->tuning, c->err);
+    }
+
+    int nnet_ctx_get_tuning(nnet_ctx* c, nnet_tuning* out)
+    {
+        if (!c || !out) return NNET_E_ARG;
+        tuning_export(c->tuning, out);
         return NNET_OK;
     }
 
@@ -2002,13 +2434,29 @@ extern "C"
         HIP_TRY(c, hipSetDevice(c->device));
         int        per_cu = 0;
         hipError_t e      = hipSuccess;
+        const bool mixed  = precision == NNET_PRECISION_MIXED;
+        const int  ppw    = c->tuning.ppw(rates_kind, eos, mixed);
         auto       occ    = [&](auto kern, int nt) { e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, 0); };
-        auto       pick   = [&](auto netc, auto eosc)
+        auto       with   = [&](auto netc, auto eosc, auto mixc)
         {
-            constexpr int NV = decltype(netc)::value, EV = decltype(eosc)::value;
-            if (precision == NNET_PRECISION_MIXED) occ(nnet_solve_kernel<NV, EV, true>, Cfg<NV>::NT);
+            constexpr int  NV = decltype(netc)::value, EV = decltype(eosc)::value;
+            constexpr bool MV = decltype(mixc)::value;
+            if constexpr (NV == 14)
+            {
+                if (ppw == 4) occ(nnet_solve_kernel<NV, EV, MV, 4>, Cfg<NV>::NT * 4);
+                else if (ppw == 2)
+                    occ(nnet_solve_kernel<NV, EV, MV, 2>, Cfg<NV>::NT * 2);
+                else
+                    occ(nnet_solve_kernel<NV, EV, MV, 1>, Cfg<NV>::NT);
+            }
             else
-                occ(nnet_solve_kernel<NV, EV, false>, Cfg<NV>::NT);
+                occ(nnet_solve_kernel<NV, EV, MV, 1>, Cfg<NV>::NT);
+        };
+        auto pick = [&](auto netc, auto eosc)
+        {
+            if (mixed) with(netc, eosc, std::true_type{});
+            else
+                with(netc, eosc, std::false_type{});
         };
         auto by_eos = [&](auto netc)
         {
@@ -2022,7 +2470,45 @@ extern "C"
         else
             by_eos(std::integral_constant<int, 87>{});
         HIP_TRY(c, e);
-        *slots = std::max(per_cu, 1) * c->n_cu;
+        *slots = std::max(per_cu, 1) * ppw * c->n_cu;
+        return NNET_OK;
+    }
+
+    int nnet_ctx_set_particles_per_workgroup(nnet_ctx* c, int rates_kind, int eos, int precision, int ppw)
+    {
+        if (!c) return NNET_E_ARG;
+        if (!(rates_kind == 14 || rates_kind == 86 || rates_kind == 87) ||
+            !(eos == NNET_EOS_IDEAL || eos == NNET_EOS_HELMHOLTZ) ||
+            !(precision == NNET_PRECISION_FP64 || precision == NNET_PRECISION_MIXED))
+            return fail(c, NNET_E_ARG, "particles per workgroup: unknown network, EOS or precision");
+        if (rates_kind == 14 ? !(ppw == 1 || ppw == 2 || ppw == 4) : ppw != 1)
+            return fail(c, NNET_E_ARG, "particles per workgroup: 1, 2 or 4 for net14, 1 for net86 and net87");
+        if (rates_kind == 14) c->tuning.ppw14[eos == NNET_EOS_HELMHOLTZ][precision == NNET_PRECISION_MIXED] = ppw;
+        return NNET_OK;
+    }
+
+    int nnet_ctx_set_fused_launch(nnet_ctx* c, int fused)
+    {
+        if (!c || (fused != 0 && fused != 1)) return NNET_E_ARG;
+        c->tuning.fused = fused;
+        return NNET_OK;
+    }
+
+    int nnet_ctx_get_fused_launch(nnet_ctx* c, int* fused)
+    {
+        if (!c || !fused) return NNET_E_ARG;
+        *fused = c->tuning.fused;
+        return NNET_OK;
+    }
+
+    int nnet_ctx_get_particles_per_workgroup(nnet_ctx* c, int rates_kind, int eos, int precision, int* ppw)
+    {
+        if (!c || !ppw) return NNET_E_ARG;
+        if (!(rates_kind == 14 || rates_kind == 86 || rates_kind == 87) ||
+            !(eos == NNET_EOS_IDEAL || eos == NNET_EOS_HELMHOLTZ) ||
+            !(precision == NNET_PRECISION_FP64 || precision == NNET_PRECISION_MIXED))
+            return fail(c, NNET_E_ARG, "particles per workgroup: unknown network, EOS or precision");
+        *ppw = c->tuning.ppw(rates_kind, eos, precision == NNET_PRECISION_MIXED);
         return NNET_OK;
     }
 

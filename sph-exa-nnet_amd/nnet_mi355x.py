@@ -47,6 +47,16 @@ class Batch(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """nnet_tuning: ppw[net14, net86, net87][eos][precision], fused."""
+    _fields_ = [("ppw", ((C.c_int * 2) * 2) * 3), ("fused", C.c_int)]
+
+    def as_dict(self):
+        nets = (14, 86, 87)
+        return {"ppw": {(nets[k], e, p): self.ppw[k][e][p] for k in range(3) for e in range(2) for p in range(2)},
+                "fused": bool(self.fused)}
+
+
 _lib = None
 
 
@@ -75,6 +85,13 @@ def lib():
         L.nnet_helm_eos_sync.argtypes = eos_args
         L.nnet_helm_table_read.argtypes = [C.c_char_p, dp]
         L.nnet_solve_slots.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip]
+        L.nnet_ctx_set_particles_per_workgroup.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.nnet_ctx_get_particles_per_workgroup.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip]
+        L.nnet_ctx_set_fused_launch.argtypes = [vp, C.c_int]
+        L.nnet_ctx_get_fused_launch.argtypes = [vp, ip]
+        L.nnet_tuning_read.argtypes = [C.c_char_p, C.POINTER(Tuning)]
+        L.nnet_ctx_load_tuning.argtypes = [vp, C.c_char_p]
+        L.nnet_ctx_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         _lib = L
     return _lib
 
@@ -84,7 +101,19 @@ EXPORTS = [
     "nnet_network_create", "nnet_network_builtin", "nnet_network_destroy", "nnet_network_reactions",
     "nnet_compute", "nnet_compute_sync", "nnet_ctx_set_max_iterations", "nnet_helm_eos", "nnet_helm_eos_sync",
     "nnet_helm_table_read", "nnet_solve_slots", "nnet_compute_noalloc",
+    "nnet_ctx_set_particles_per_workgroup", "nnet_ctx_get_particles_per_workgroup",
+    "nnet_ctx_set_fused_launch", "nnet_ctx_get_fused_launch", "nnet_tuning_read", "nnet_ctx_load_tuning",
+    "nnet_ctx_get_tuning",
 ]
+
+
+def tuning_read(path):
+    """Parse an autotuner table on the host (no device): the launch parameters as a dict."""
+    t = Tuning()
+    rc = lib().nnet_tuning_read(path.encode(), C.byref(t))
+    if rc != 0:
+        raise NnetError(f"nnet_tuning_read({path}): {rc}")
+    return t.as_dict()
 
 
 def helm_table_read(path):
@@ -145,6 +174,34 @@ class Context:
 
     def set_max_iterations(self, m):
         self.check(lib().nnet_ctx_set_max_iterations(self.h, m), "set_max_iterations")
+
+    def set_particles_per_workgroup(self, net, eos, precision, ppw):
+        """Particles per workgroup of the solve launches (net14: 1, 2 or 4; net86/net87: 1)."""
+        self.check(lib().nnet_ctx_set_particles_per_workgroup(self.h, net, eos, precision, ppw),
+                   "set_particles_per_workgroup")
+
+    def load_tuning(self, path):
+        self.check(lib().nnet_ctx_load_tuning(self.h, path.encode()), "load_tuning")
+
+    def tuning(self):
+        t = Tuning()
+        self.check(lib().nnet_ctx_get_tuning(self.h, C.byref(t)), "tuning")
+        return t.as_dict()
+
+    def set_fused_launch(self, fused):
+        """Batches of several networks in one call: one fused launch (True) or one launch per batch."""
+        self.check(lib().nnet_ctx_set_fused_launch(self.h, int(bool(fused))), "set_fused_launch")
+
+    def fused_launch(self):
+        out = C.c_int()
+        self.check(lib().nnet_ctx_get_fused_launch(self.h, C.byref(out)), "fused_launch")
+        return bool(out.value)
+
+    def particles_per_workgroup(self, net, eos, precision):
+        out = C.c_int()
+        self.check(lib().nnet_ctx_get_particles_per_workgroup(self.h, net, eos, precision, C.byref(out)),
+                   "particles_per_workgroup")
+        return out.value
 
     def make_batch(self, net, state, eos=EOS_IDEAL, mu=10.0, Z=None, precision=PRECISION_FP64):
         """state: dict of torch device tensors rho, rho_m1, Y (d x n), temp, dt, status, substeps, newton
