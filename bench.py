@@ -298,9 +298,10 @@ def main():
     ap.add_argument("--workload", default="solve", choices=["solve", "helm-eos"],
                     help="solve (default): computeNuclearReactions; helm-eos: the post-step EOS refresh "
                          "computeHelmEOS over the same particle set")
-    ap.add_argument("--cpu-sample", type=int, default=65536,
+    ap.add_argument("--cpu-sample", type=int, default=262144,
                     help="particles of the CPU-baseline sample: a stratified subsample of the device state at the "
-                         "start of the first timed step, one hydro step on the host's physical cores (~10-30 s)")
+                         "start of the first timed step, one hydro step on the host's physical cores (C3: ~5 s "
+                         "on 16 cores, ~80 core-seconds)")
     ap.add_argument("--max-iter", type=int, default=10000,
                     help="Newton-iteration cap per particle and step (nnet_ctx_set_max_iterations; the oracle and "
                          "the CPU baseline use the same cap)")

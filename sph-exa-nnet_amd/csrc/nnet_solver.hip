@@ -1,20 +1,21 @@
 // libnnet_mi355x: MI355X (gfx950) implementation of SPH-EXA-nnet's batched implicit nuclear
 // network solve behind the C-ABI of include/nnet_mi355x.h.
 //
-// Execution model (FP64 path): one particle per workgroup, the workgroup's lanes share the work
-// of one Newton iteration:
+// Execution model (FP64 path): the lanes of one particle group (a workgroup of four wavefronts for
+// net86 / net87, one wavefront for net14) share the work of one Newton iteration:
 //   rates      one lane per fitted rate / hand-fitted group / species (nnet_dev_rates.hpp)
-//   assembly   the (d+1)x(d+1) Newton matrix is built directly into registers: lane (tr, tc) of
-//              a TR x TC lane grid owns rows tr + TR*a and columns tc + TC*b.  Each entry is
-//              accumulated from a host-precomputed ordered contribution list, so the value is
-//              the one the reference's scatter loops (nuclear_net.hpp:347-438) produce.
-//   LU         right-looking unpivoted elimination on the register-resident matrix with the
-//              reference's per-element update order (eigen.hpp:179-206); the normalised pivot
-//              rows are staged in LDS, which also serves the back substitution.
+//   assembly   Jacobian entries summed from host-precomputed ordered contribution lists (the
+//              values the reference's scatter loops, nuclear_net.hpp:347-438, produce), then the
+//              (d+1)x(d+2) augmented Newton matrix is filled directly into registers as 16x16
+//              tiles in the v_mfma_f64_16x16x4f64 result layout.
+//   LU         right-looking unpivoted elimination on the register-resident tiles, blocks of
+//              four pivots, rank-4 trailing updates on MFMA (eigen.hpp:179-206); the normalised
+//              pivot rows are staged in LDS, which also serves the back substitution.
 //   controller the substep/Newton state machine of nuclear_net.hpp:692-959 is replicated in
 //              every lane (uniform control flow) with exact replay-stall detection.
-// A grid of one workgroup per particle lets the hardware dispatcher balance particles of very
-// different stiffness across the CUs.
+// Persistent workgroups (as many as are resident at once) take particles from a device work queue,
+// longest first when the caller passes a cost hint, so particles of very different stiffness
+// share the CUs without a tail of idle workgroups.
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
@@ -1139,10 +1140,16 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
             };
             // the lane's register slots (tile placement of TileLU) filled from the Jacobian
             // non-zeros, which are numbered lane-major in slot order
+            // Every slot first takes its Jacobian-block value (rows and columns 1 .. N-1, zero
+            // elsewhere) without branches; then the few slots of column 0 (dY/dT), column N (the
+            // right-hand side) and row 0 (temperature) are overwritten where they sit: column 0 in
+            // register lane column 0 of tile column 0, column N in lane column N & 15 of tile column
+            // N >> 4, row 0 in register 0 of lane row 0 of tile row 0.  Each slot gets the value
+            // entry() gives it.
             auto fill = [&](auto& Mt) __attribute__((always_inline))
             {
                 const uint64_t mask = (uint64_t)lmap.x | ((uint64_t)lmap.y << 32);
-                int            nzi  = (int)lmap.z; // id of this lane's next non-zero
+                const int      base = (int)lmap.z; // id of this lane's first non-zero
 #pragma unroll
                 for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
@@ -1150,23 +1157,50 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
 #pragma unroll
                         for (int v = 0; v < 4; ++v)
                         {
-                            const int    slot = (ti * TI + tj) * 4 + v;
-                            const int    ri = tile_row(wr + WR * ti, lr, v), ci = tile_col(wc + WR * tj, lc);
-                            const bool   nz = (mask >> slot) & 1;
-                            const double J  = nz ? sJv[nzi] : 0.;
-                            nzi += nz ? 1 : 0;
-                            double x = 0.;
-                            // row 0 (temperature) can only sit in register 0 of tile row 0
-                            if (ti == 0 && v == 0 && (wr | lr) == 0)
-                            {
-                                if (ci <= N) x = entry(0, ci, 0.);
-                                if constexpr (MIX)
-                                    if (ci <= N) sRow0[ci] = x;
-                            }
-                            else if (ri < N && ci <= N)
-                                x = entry(ri, ci, J);
-                            Mt[ti][tj][v] = x;
+                            const int  slot = (ti * TI + tj) * 4 + v;
+                            const int  ri = tile_row(wr + WR * ti, lr, v), ci = tile_col(wc + WR * tj, lc);
+                            const bool nz = (mask >> slot) & 1;
+                            // the lane's non-zeros are numbered in slot order
+                            const int    nzi = base + (int)__popcll(mask & ((1ull << slot) - 1ull));
+                            const double Jl  = sJv[nz ? nzi : 0];
+                            const double J   = nz ? Jl : 0.;
+                            const double x   = (ci == ri) ? 1. - thdt * J : mthdt * J;
+                            Mt[ti][tj][v]    = (ri >= 1 && ri < N && ci >= 1 && ci < N) ? x : 0.;
                         }
+                // column 0: dY/dT
+                if (wc == 0 && lc == 0)
+#pragma unroll
+                    for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                        {
+                            const int ri = tile_row(wr + WR * ti, lr, v);
+                            if (ri >= 1 && ri < N) Mt[ti][0][v] = entry(ri, 0, 0.);
+                        }
+                // column N: the right-hand side
+                constexpr int JN = N >> 4;
+                if (wc == JN % WR && lc == (N & 15))
+#pragma unroll
+                    for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                        {
+                            const int ri = tile_row(wr + WR * ti, lr, v);
+                            if (ri >= 1 && ri < N) Mt[ti][JN / WR][v] = entry(ri, N, 0.);
+                        }
+                // row 0 (temperature)
+                if (wr == 0 && lr == 0)
+#pragma unroll
+                    for (int tj = 0; tj < TI; ++tj)
+                    {
+                        const int ci = tile_col(wc + WR * tj, lc);
+                        if (ci <= N)
+                        {
+                            const double x = entry(0, ci, 0.);
+                            Mt[0][tj][0]   = x;
+                            if constexpr (MIX) sRow0[ci] = x;
+                        }
+                    }
             };
             auto solve_fp64 = [&]() __attribute__((always_inline))
             {
@@ -1487,14 +1521,27 @@ __global__ void __launch_bounds__(FUSE_NT, 2) nnet_solve_fused(FusedDev fd, Step
         for (int w = threadIdx.x; w < nd.img_words; w += FUSE_NT)
             simg[w] = nd.img[w];
         __syncthreads();
-        if (P.kind == 87) solve_particles<87, EOSK, MIX, false>(nd, ev, bv, sp, sm, simg, s_pf, threadIdx.x);
+        // the lane index enters each network's code through an empty asm statement, so the
+        // compiler forms each network's lane-dependent values inside its own branch instead of
+        // hoisting common ones above the branches, where they would stay live through all three
+        // (fused Helmholtz FP64 kernel: 72 -> 25 spilled VGPRs)
+        int t = threadIdx.x;
+        if (P.kind == 87)
+        {
+            asm volatile("" : "+v"(t));
+            solve_particles<87, EOSK, MIX, false>(nd, ev, bv, sp, sm, simg, s_pf, t);
+        }
         else if (P.kind == 86)
-            solve_particles<86, EOSK, MIX, false>(nd, ev, bv, sp, sm, simg, s_pf, threadIdx.x);
+        {
+            asm volatile("" : "+v"(t));
+            solve_particles<86, EOSK, MIX, false>(nd, ev, bv, sp, sm, simg, s_pf, t);
+        }
         else
         {
-            const int g = threadIdx.x >> 6;
+            asm volatile("" : "+v"(t));
+            const int g = t >> 6;
             solve_particles<14, EOSK, MIX, true>(nd, ev, bv, sp, sm + g * Lds<14, MIX>::total, simg, s_pf + 2 * g,
-                                                 threadIdx.x & 63);
+                                                 t & 63);
         }
     }
 }

@@ -159,11 +159,17 @@ void run()
             IN(tag + "sRow0[c]", L::oRow0 + c, L::oRow0, L::oPan);
 
     // ---- matrix fill: register slots, non-zero ids, entry() operands ----
+    // Jacobian block for every slot (index selected before the load), then the column-0,
+    // column-N and row-0 patches; every matrix entry (r, c), r < N, c <= N, must be written by
+    // exactly one of: the block (1 <= r, c < N) or one patch.
+    std::vector<int> written((size_t)N * (N + 1), 0);
+    auto             mark = [&](int r, int c) { if (r >= 0 && r < N && c >= 0 && c <= N) ++written[(size_t)r * (N + 1) + c]; };
+    constexpr int    JN   = N >> 4;
     for (int tid = 0; tid < NT; ++tid)
     {
         const int      lr = (tid & 63) >> 4, lc = tid & 15, wr = (tid >> 6) / WR, wc = (tid >> 6) % WR;
         const uint64_t mask = (uint64_t)im.lanemap[tid * 4] | ((uint64_t)im.lanemap[tid * 4 + 1] << 32);
-        int            nzi  = (int)im.lanemap[tid * 4 + 2];
+        const int      base = (int)im.lanemap[tid * 4 + 2];
         for (int ti = 0; ti < TI; ++ti)
             for (int tj = 0; tj < TI; ++tj)
                 for (int v = 0; v < 4; ++v)
@@ -171,19 +177,51 @@ void run()
                     const int  slot = (ti * TI + tj) * 4 + v;
                     const int  ri = tile_row(wr + WR * ti, lr, v), ci = tile_col(wc + WR * tj, lc);
                     const bool nz = (mask >> slot) & 1;
-                    // J = nz ? sJv[nzi] : 0 -- a select: the address must be in LDS either way
+                    const int  nzi = base + __builtin_popcountll(mask & ((1ull << slot) - 1ull));
                     if (nz) IN(tag + "fill sJv[nzi]", L::oJv + nzi, L::oJv, L::oJv + nnz);
-                    IN(tag + "fill sJv[nzi] (select)", L::oJv + nzi, 0, total);
-                    nzi += nz ? 1 : 0;
-                    if (ti == 0 && v == 0 && (wr | lr) == 0)
-                    {
-                        if (ci <= N && ci >= 1 && ci < N) IN(tag + "entry sBE[ci-1]", L::oBE + ci - 1, L::oBE, L::oDYdot);
-                        if (MIX && ci <= N) IN(tag + "fill sRow0[ci]", L::oRow0 + ci, L::oRow0, L::oPan);
-                    }
-                    else if (ri < N && ci <= N)
-                        IN(tag + "entry row operands", L::oDYdT + ri - 1, L::oDYdT, L::oCorr);
+                    IN(tag + "fill sJv[nz ? nzi : 0]", L::oJv + (nz ? nzi : 0), L::oJv, L::oJv + nnz);
+                    if (ri >= 1 && ri < N && ci >= 1 && ci < N) mark(ri, ci);
                 }
+        if (wc == 0 && lc == 0) // column 0
+            for (int ti = 0; ti < TI; ++ti)
+                for (int v = 0; v < 4; ++v)
+                {
+                    const int ri = tile_row(wr + WR * ti, lr, v);
+                    if (ri >= 1 && ri < N)
+                    {
+                        IN(tag + "column 0 sDYT[ri-1]", L::oDYdT + ri - 1, L::oDYdT, L::oCorr);
+                        mark(ri, 0);
+                    }
+                }
+        if (wc == JN % WR && lc == (N & 15)) // column N
+            for (int ti = 0; ti < TI; ++ti)
+                for (int v = 0; v < 4; ++v)
+                {
+                    const int ri = tile_row(wr + WR * ti, lr, v);
+                    if (ri >= 1 && ri < N)
+                    {
+                        IN(tag + "column N sDYd[ri-1]", L::oDYdot + ri - 1, L::oDYdot, L::oDYdT);
+                        IN(tag + "column N sDYT[ri-1]", L::oDYdT + ri - 1, L::oDYdT, L::oCorr);
+                        IN(tag + "column N sCorr[ri-1]", L::oCorr + ri - 1, L::oCorr, L::oEv);
+                        check(tile_col(wc + WR * (JN / WR), lc) == N, tag + "column N tile", ri, 0, 1);
+                        mark(ri, N);
+                    }
+                }
+        if (wr == 0 && lr == 0) // row 0
+            for (int tj = 0; tj < TI; ++tj)
+            {
+                const int ci = tile_col(wc + WR * tj, lc);
+                if (ci <= N)
+                {
+                    if (ci >= 1 && ci < N) IN(tag + "row 0 sBE[ci-1]", L::oBE + ci - 1, L::oBE, L::oDYdot);
+                    if (MIX) IN(tag + "fill sRow0[ci]", L::oRow0 + ci, L::oRow0, L::oPan);
+                    mark(0, ci);
+                }
+            }
     }
+    for (int r = 0; r < N; ++r)
+        for (int c = 0; c <= N; ++c)
+            check(written[(size_t)r * (N + 1) + c] == 1, tag + "matrix entry written once", r * (N + 1) + c, 1, 2);
 
     // ---- tile LU: panel publish / factor steps, packed rows, multipliers ----
     const int panEnd = 16 * NPAD;
