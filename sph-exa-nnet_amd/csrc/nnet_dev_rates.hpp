@@ -2,8 +2,10 @@
 // cooperatively by the lanes of one workgroup (one particle per workgroup):
 //   * one lane per fitted rate (net14: 10, net86/87: 150) computes the forward rate, its
 //     reverse from the partition-function ratio, and both T-derivatives;
-//   * six lanes compute the six hand-fitted rates (3-alpha, 12C+12C, 12C+16O, 16O+16O,
-//     12C(a,g), 16O(a,g)) with reverses and derivatives;
+//   * the six hand-fitted rates (3-alpha, 12C+12C, 12C+16O, 16O+16O, 12C(a,g), 16O(a,g)) with
+//     reverses and derivatives: net86 / net87, one group per lane on lanes of different
+//     wavefronts (the serial forms); net14 (one wavefront per particle), their exp / pow values
+//     one per lane of the wavefront, then combined (the lane-parallel form);
 //   * one lane per species computes the Coulomb-corrected binding energy and the screening
 //     chemical potential mu/kT.
 // A second phase applies screening per rate and scatters into the reaction-list order.
@@ -294,6 +296,265 @@ __device__ __forceinline__ void special_o16ag(double T, double& r, double& rl, d
         const double vA  = -54.903255 * q.t9i;
         const double dvA = 54.903255 * q.t9i2;
         drl              = 5.65e10 * exp(vA) * (dr * q.t932 + 1.5 * r * q.t912 + r * q.t932 * dvA) * 1.e-9;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// The six groups above, lane-parallel over one wavefront (net14, whose one wavefront otherwise
+// runs the six serial forms one after another).  Every exp / pow the groups evaluate is a
+// function of T only: lane k of the wavefront evaluates value k -- level 1: 36 exps and 13 pows
+// of arguments built from the t9 powers; level 2: 6 exps whose arguments need a level-1 pow --
+// and every lane then combines the values into the groups' outputs with the serial forms'
+// expressions, each exp / pow replaced by its value.  Each value comes from the same argument
+// expression through the same function, so the results are bit-identical to the serial forms';
+// the wavefront runs one exp pass and one pow pass instead of six chains of them.
+// ---------------------------------------------------------------------------------------------
+// Level-1 value k < spx::N1: exp(a) or pow(a, b) with the reference's argument expression.
+__device__ __forceinline__ double special_level1(int k, const RatePow& p, const DerivPow& q)
+{
+    double a = 0., b = 1.; // exp(a) for k < NE1, pow(a, b) otherwise
+    switch (k)
+    {
+    // group 0, triple alpha: r2abe, rbeac, r, rev (RatePow), then the derivative exps (DerivPow)
+    case 0: a = -1.0663 * p.t9i; break;
+    case 1: a = -13.49 * p.t9i13 - p.t92 / 0.009604; break;
+    case 2: a = -3.3364 * p.t9i; break;
+    case 3: a = -23.57 * p.t9i13 - p.t92 / 0.055225; break;
+    case 4: a = -24.811 * p.t9i; break;
+    case 5: a = -84.419412e0 * p.t9i; break;
+    case 6: a = -1.0663 * q.t9i; break;                           // vB
+    case 7: a = -13.49 * q.t9i13 - q.t92 * 104.123282; break;     // vC
+    case 8: a = -3.3364 * q.t9i; break;                           // vE
+    case 9: a = -23.57 * q.t9i13 - 18.10774106 * q.t92; break;    // vF
+    case 10: a = -24.811 * q.t9i; break;                          // vA of dr
+    case 11: a = -84.419412 * q.t9i; break;                       // vA of drl
+    // group 3, 16O + 16O
+    case 12: a = -135.93 * p.t9i13 - 0.629 * p.t923 - 0.445 * p.t943 + 0.0103 * p.t92; break;
+    case 13: a = -135.93e0 * q.t9i13 - .629 * q.t923 - .445 * q.t943 + .0103 * q.t92; break;
+    // group 4, 12C(a,g)
+    case 14: a = -32.120 * p.t9i13 - p.t92 / 12.222016; break;
+    case 15: a = -32.120 * p.t9i13; break;
+    case 16: a = -27.499 * p.t9i; break;
+    case 17: a = -15.541 * p.t9i; break;
+    case 18: a = -83.108047 * p.t9rm1; break;
+    case 19:
+    {
+        const double s1 = q.t9 / 3.496;
+        a               = -32.120 * q.t9i13 - s1 * s1; // vC
+        break;
+    }
+    case 20: a = -32.120 * q.t9i13; break; // vE
+    case 21: a = -27.499 * q.t9i; break;   // vF
+    case 22: a = -15.541 * q.t9i; break;   // vG
+    case 23: a = -83.108047 * q.t9i; break;
+    // group 5, 16O(a,g)
+    case 24: a = -39.757 * p.t9i13 - p.t92 / 2.515396; break;
+    case 25: a = -10.297 * p.t9i; break;
+    case 26: a = -12.226 * p.t9i; break;
+    case 27: a = -20.093 * p.t9i; break;
+    case 28: a = -54.93807 * p.t9rm1; break;
+    case 29:
+    {
+        const double s1 = q.t9 / 1.586;
+        a               = -39.757 * q.t9i13 - s1 * s1; // vB
+        break;
+    }
+    case 30: a = -10.297 * q.t9i; break; // vC
+    case 31: a = -12.226 * q.t9i; break; // vD
+    case 32: a = -20.093 * q.t9i; break; // vE
+    case 33: a = -54.903255 * q.t9i; break;
+    // group 2, 12C + 16O: the exps that need no pow
+    case 34:
+    {
+        const double t9ap = p.t9 / (1. + 0.055 * p.t9);
+        a                 = -0.18 * (t9ap * t9ap);
+        break;
+    }
+    case 35:
+    {
+        const double vA = q.t9 / (1. + .055 * q.t9);
+        a               = -.18 * vA * vA; // vC
+        break;
+    }
+    // group 1, 12C + 12C: the pows
+    case 36: a = p.t9 / (1. + 0.0396 * p.t9), b = 1. / 3.; break;
+    case 37: a = p.t9 / (1. + 0.0396 * p.t9), b = 5. / 6.; break;
+    case 38: a = q.t9 / (1. + .0396 * q.t9), b = 5. / 6.; break;
+    case 39: a = q.t9 / (1. + .0396 * q.t9), b = -1. / 3.; break;
+    case 40: a = q.t9 / (1. + .0396 * q.t9), b = -4. / 3.; break;
+    case 41: a = q.t9 / (1. + .0396 * q.t9), b = -1. / 6.; break;
+    // group 2: the pows
+    case 42: a = p.t9 / (1. + 0.055 * p.t9), b = 1. / 3.; break;
+    case 43: a = p.t9 / (1. + 0.055 * p.t9), b = 5. / 6.; break;
+    case 44: a = q.t9 / (1. + .055 * q.t9), b = 5. / 6.; break;
+    case 45: a = q.t9 / (1. + .055 * q.t9), b = -1. / 3.; break;
+    case 46: a = q.t9 / (1. + .055 * q.t9), b = 2. / 3.; break;
+    case 47: a = q.t9 / (1. + .055 * q.t9), b = -4. / 3.; break;
+    case 48: a = q.t9 / (1. + .055 * q.t9), b = -1. / 6.; break;
+    default: break;
+    }
+    return k < spx::NE1 ? exp(a) : pow(a, b);
+}
+
+// Level-2 value k < spx::NE2 from the level-1 values v1(j) (wavefront-uniform).
+template<class V1>
+__device__ __forceinline__ double special_level2(int k, const RatePow& p, const DerivPow& q, const V1& v1)
+{
+    double a = 0.;
+    switch (k)
+    {
+    case 0: a = -84.165 / v1(36) - 2.12e-03 * p.t93; break;               // c12c12 r: t9a13 = pow(t9a, 1/3)
+    case 1: a = -84.165 * v1(39) - 2.12e-3 * q.t93; break;                // c12c12 vB
+    case 2: a = -106.594 / v1(42); break;                                // c12o16 r
+    case 3: a = 2.562 * (v1(42) * v1(42)); break;                        // c12o16 r: t9a23p
+    case 4: a = 2.562 * v1(46); break;                                   // c12o16 vD
+    case 5: a = -106.594 * v1(45); break;                                // c12o16 vB
+    default: break;
+    }
+    return exp(a);
+}
+
+// The six groups' outputs from the level-1 and level-2 values (the reference's expressions,
+// each exp / pow replaced by its value), handed to out(group, r, rl, dr, drl) group by group
+// (rl, drl only for groups 0, 4, 5; zero otherwise).
+template<class V1, class V2, class Out>
+__device__ __forceinline__ void special_combine(double T, const RatePow& p, const DerivPow& q, const V1& v1,
+                                                const V2& v2, const Out& out)
+{
+    // group 0: triple alpha
+    {
+        double gr = 0, grl = 0, gdr = 0, gdrl = 0;
+        const double r2abe = (7.40e+05 * p.t9i32) * v1(0) +
+                             4.164e+09 * p.t9i23 * v1(1) *
+                                 (1. + 0.031 * p.t913 + 8.009 * p.t923 + 1.732 * p.t9 + 49.883 * p.t943 + 27.426 * p.t953);
+        const double rbeac = (130. * p.t9i32) * v1(2) +
+                             2.510e+07 * p.t9i23 * v1(3) *
+                                 (1. + 0.018 * p.t913 + 5.249 * p.t923 + 0.650 * p.t9 + 19.176 * p.t943 + 6.034 * p.t953);
+        double r;
+        if (T > 8e7)
+            r = 2.90e-16 * (r2abe * rbeac) + 0.1 * 1.35e-07 * p.t9i32 * v1(4);
+        else
+            r = 2.90e-16 * (r2abe * rbeac) *
+                    (0.01 + 0.2 * (1. + 4. * exp(-pow(0.025 * p.t9i, 3.263))) /
+                                (1. + 4. * exp(-pow(p.t9 / 0.025, 9.227)))) +
+                0.1 * 1.35e-07 * p.t9i32 * v1(4);
+        const double rev = 2.e20 * v1(5);
+        gr           = r;
+        grl          = r * rev * p.t93;
+        const double vD  = 1. + .031 * q.t913 + 8.009 * q.t923 + 1.732 * q.t9 + 49.883 * q.t943 + 27.426 * q.t953;
+        const double vG  = 1. + .018 * q.t913 + 5.249 * q.t923 + .650 * q.t9 + 19.176 * q.t943 + 6.034 * q.t953;
+        const double dvA = 24.811 * q.t9i2;
+        const double dvB = 1.0663 * q.t9i2;
+        const double dvC = q.t9i43 * 13.49 / 3. - 208.246564 * q.t9;
+        const double dvD = (q.t9i23 * .031 + q.t9i13 * 16.018 + 5.196 + q.t913 * 199.532 + q.t923 * 137.13) / 3.;
+        const double dvE = 3.3364 * q.t9i2;
+        const double dvF = q.t9i43 * 23.57 / 3. - 36.21548212 * q.t9;
+        const double dvG = (q.t9i23 * .018 + q.t9i13 * 10.498 + 1.950 + q.t913 * 76.704 + q.t923 * 30.17) / 3.;
+        const double eB = v1(6), eC = v1(7), eE = v1(8), eF = v1(9);
+        const double r2abe_ = 7.4e5 * q.t9i32 * eB + 4.164e9 * q.t9i23 * vD * eC;
+        const double rbeac_ = 130. * q.t9i32 * eE + 2.510e7 * q.t9i23 * vG * eF;
+        const double dr2abe = eB * (-1.11e6 * q.t9i52 + 7.4e5 * q.t9i32 * dvB) +
+                              4.164e9 * eC * (-q.t9i53 * vD * 2. / 3. + q.t9i23 * dvC * vD + q.t9i23 * dvD);
+        const double drbeac = eE * (-195. * q.t9i52 + 130. * q.t9i32 * dvE) +
+                              2.510e7 * eF * (-2. * q.t9i53 * vG / 3. + q.t9i23 * dvF * vG + q.t9i23 * dvG);
+        const double dr = (2.90e-16 * (dr2abe * rbeac_ + r2abe_ * drbeac) + 1.35e-8 * v1(10) * (-1.5 * q.t9i52 + q.t9i32 * dvA)) *
+                          1.e-9;
+        gdr           = dr;
+        const double dvA2 = 84.419412 * q.t9i2;
+        gdrl          = 2.00e20 * v1(11) * q.t93 * (dvA2 * r + 3. * q.t9i * r + dr) * 1.e-9;
+        out(0, gr, grl, gdr, gdrl);
+    }
+    // group 1: 12C + 12C
+    {
+        double gr = 0, grl = 0, gdr = 0, gdrl = 0;
+        const double t9a56 = v1(37);
+        gr             = 4.27e+26 * t9a56 * p.t9i32 * v2(0);
+        const double vA    = q.t9 / (1. + .0396 * q.t9);
+        const double vA56  = v1(38);
+        const double dvA   = vA * vA * q.t9i2;
+        const double dvB   = 28.055 * dvA * v1(40) - 6.36e-3 * q.t92;
+        gdr = 4.27e26 * q.t9i32 * v2(1) * (v1(41) * dvA * 5. / 6. - 1.5 * vA56 * q.t9i + vA56 * dvB) * 1.e-9;
+        out(1, gr, grl, gdr, gdrl);
+    }
+    // group 2: 12C + 16O
+    {
+        double gr = 0, grl = 0, gdr = 0, gdrl = 0;
+        if (T >= 5e8)
+        {
+            const double t9a56ap = v1(43);
+            gr = 1.72e+31 * t9a56ap * p.t9i32 * v2(2) / (v1(34) + 1.06e-03 * v2(3));
+        }
+        const double t9 = T * 1.0e-09;
+        if (t9 > .5)
+        {
+            const double vA   = q.t9 / (1. + .055 * q.t9);
+            const double vA56 = v1(44);
+            const double eC = v1(35), eD = v2(4);
+            const double val  = eC + 1.06e-3 * eD;
+            const double dvA  = vA * vA * q.t9i2;
+            const double dvB  = 106.594 * v1(47) * dvA / 3.;
+            const double dvC  = -.36 * vA * dvA;
+            const double dvD  = 1.708 * dvA * v1(45);
+            const double dval = dvC * eC + 1.06e-3 * dvD * eD;
+            gdr = 1.72e31 * q.t9i32 * (v1(48) * dvA * 5. / 6. - 1.5 * vA56 * q.t9i + vA56 * (dvB - dval / val)) *
+                      v2(5) / val * 1.e-9;
+        }
+        out(2, gr, grl, gdr, gdrl);
+    }
+    // group 3: 16O + 16O
+    {
+        double gr = 0, grl = 0, gdr = 0, gdrl = 0;
+        gr           = 7.10e+36 * p.t9i23 * v1(12);
+        const double dvA = 45.31 * q.t9i43 - .629 * q.t9i13 * 2. / 3. - .445 * q.t913 * 4. / 3. + .0206 * q.t9;
+        gdr          = 7.10e36 * v1(13) * q.t9i23 * (-q.t9i * 2. / 3. + dvA) * 1.e-9;
+        out(3, gr, grl, gdr, gdrl);
+    }
+    // group 4: 12C(a,g)16O
+    {
+        double gr = 0, grl = 0, gdr = 0, gdrl = 0;
+        const double b1 = 1. + 0.0489 * p.t9i23, b2 = 1. + 0.2654 * p.t9i23;
+        const double r  = 1.04e+08 / (p.t92 * (b1 * b1)) * v1(14) + 1.76e+08 / (p.t92 * (b2 * b2)) * v1(15) +
+                         1.25e+03 * p.t9i32 * v1(16) + 1.43e-02 * p.t95 * v1(17);
+        gr           = r;
+        grl          = r * 5.13e+10 * p.t9r32 * v1(18);
+        const double qb1 = 1. + .0489 * q.t9i23;
+        const double vB  = qb1 * qb1;
+        const double qb2 = 1. + .2654 * q.t9i23;
+        const double vD  = qb2 * qb2;
+        const double dvB = -(2. + .0978 * q.t9i23) * (.0326 * q.t9i53);
+        const double dvC = 32.120 * q.t9i43 / 3. - 2. * q.t9 / (3.496 * 3.496);
+        const double dvD = -(2. + .5308 * q.t9i23) * (.5308 * q.t9i53 / 3.);
+        const double dvE = 32.120 * q.t9i43 / 3.;
+        const double dvF = 27.499 * q.t9i2;
+        const double dvG = 15.541 * q.t9i2;
+        const double dr  = (1.04e8 * v1(19) * q.t9i2 * (-2. * q.t9i + dvC - dvB / vB) / vB +
+                           1.76e8 * v1(20) * q.t9i2 * (-2. * q.t9i + dvE - dvD / vD) / vD +
+                           1.25e3 * v1(21) * (-1.5 * q.t9i52 + dvF * q.t9i32) + 1.43e-2 * v1(22) * (5. * q.t94 + dvG * q.t95)) *
+                          1.e-9;
+        gdr          = dr;
+        const double dvA = 83.108047 * q.t9i2;
+        gdrl         = 5.13e10 * v1(23) * (dr * q.t932 + r * 1.5 * q.t912 + r * q.t932 * dvA) * 1.e-9;
+        out(4, gr, grl, gdr, gdrl);
+    }
+    // group 5: 16O(a,g)20Ne
+    {
+        double gr = 0, grl = 0, gdr = 0, gdrl = 0;
+        const double r = (9.37e+09 * p.t9i23 * v1(24) + 62.1 * p.t9i32 * v1(25) + 538. * p.t9i32 * v1(26) +
+                          13. * p.t92 * v1(27));
+        gr           = r;
+        grl          = r * 5.65e+10 * p.t9r32 * v1(28);
+        const double dvB = 39.757 * q.t9i43 / 3. - 2. * q.t9 / (1.586 * 1.586);
+        const double dvC = 10.297 * q.t9i2;
+        const double dvD = 12.226 * q.t9i2;
+        const double dvE = 20.093 * q.t9i2;
+        const double dr  = (9.37e9 * v1(29) * (-q.t9i53 * 2. / 3. + q.t9i23 * dvB) +
+                           62.1 * v1(30) * (-1.5 * q.t9i52 + q.t9i32 * dvC) + 538. * v1(31) * (-1.5 * q.t9i52 + q.t9i32 * dvD) +
+                           13. * v1(32) * (2. * q.t9 + q.t92 * dvE)) *
+                          1.e-9;
+        gdr          = dr;
+        const double dvA = 54.903255 * q.t9i2;
+        gdrl         = 5.65e10 * v1(33) * (dr * q.t932 + 1.5 * r * q.t912 + r * q.t932 * dvA) * 1.e-9;
+        out(5, gr, grl, gdr, gdrl);
     }
 }
 

@@ -9,6 +9,14 @@ namespace nnet_amd
 constexpr int MAXQ  = 3;  // reactant entries per reaction
 constexpr int MAXSP = 88; // species + 1 of the largest network
 
+// The exp / pow values of the six hand-fitted rate groups, evaluated one per lane of a wavefront:
+// level 1, NE1 exps and NP1 pows; level 2, NE2 exps of level-1 results.  They pass through the
+// Jacobian array sJv (N1 + NE2 entries), which is rewritten only after the rate phase.
+namespace spx
+{
+constexpr int NE1 = 36, NP1 = 13, N1 = NE1 + NP1, NE2 = 6;
+}
+
 // Rate slots written by the six hand-fitted rate groups (3-alpha forward/reverse, 12C+12C,
 // 12C+16O, 16O+16O, 12C(a,g) forward/reverse, 16O(a,g) forward/reverse).
 struct SpecialIdx

@@ -592,7 +592,7 @@ __device__ __forceinline__ int special_task(int tid)
     // placements that measured the shortest rate phase (its slowest wavefront sets its length)
     if (NET == 86) return (tid == 32 || tid == 33) ? tid - 32 : (tid == 96 || tid == 97) ? tid - 94 : tid == 160 ? 4 : tid == 98 ? 5 : -1;
     if (NET == 87) return (tid == 32 || tid == 33) ? tid - 32 : (tid == 96 || tid == 97) ? tid - 94 : tid == 160 ? 4 : tid == 233 ? 5 : -1;
-    return tid >= 32 && tid < 38 ? tid - 32 : -1;
+    return -1; // net14: the lane-parallel form on its one wavefront
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -841,42 +841,88 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
                         sMu[s] = mu;
                     }
                 }
-                // hand-fitted rates: six groups on lanes of different wavefronts
-                const int g = special_task<NET>(tid);
-                if (g >= 0)
+                // hand-fitted rates.  net14: the groups' exp / pow values spread over the lanes of
+                // its one wavefront, then combined in every lane; net86 / net87: one group per lane
+                // on lanes of different wavefronts (measured faster there than the lane-parallel
+                // form on one of their wavefronts)
+                if constexpr (NET == 14)
                 {
-                    double r = 0, rl = 0, dr = 0, drl = 0;
-                    if (g == 0)
+                    if ((tid >> 6) == 0)
                     {
-                        special_triple_alpha(Tg, r, rl, dr, drl);
-                        sEff[nd.sp.r3a_e] = r, sDeff[nd.sp.r3a_e] = dr, sL[nd.sp.r3a_l] = rl, sDl[nd.sp.r3a_l] = drl;
+                        // the values pass through the Jacobian array, which is rewritten only after
+                        // this phase: sJv[0 .. N1) level 1, sJv[N1 .. N1 + NE2) level 2
+                        static_assert(C::NNZ >= spx::N1 + spx::NE2, "hand-fitted values fit the Jacobian array");
+                        const int      k = tid & 63;
+                        const RatePow  p(Tg);
+                        const DerivPow q(Tg);
+                        {
+                            const double t1 = special_level1(k, p, q);
+                            if (k < spx::N1) sJv[k] = t1;
+                        }
+                        psync<true>(); // one wavefront
+                        auto v1 = [&](int j) __attribute__((always_inline)) -> double { return sJv[j]; };
+                        {
+                            const double t2 = special_level2(k, p, q, v1);
+                            if (k < spx::NE2) sJv[spx::N1 + k] = t2;
+                        }
+                        psync<true>();
+                        auto v2 = [&](int j) __attribute__((always_inline)) -> double { return sJv[spx::N1 + j]; };
+                        // lane 0 stores each group's rate (and reverse) with its derivative
+                        auto store = [&](int g, double r, double rl, double dr, double drl) __attribute__((always_inline))
+                        {
+                            if (k != 0) return;
+                            const int e = g == 0 ? nd.sp.r3a_e : g == 1 ? nd.sp.r24 : g == 2 ? nd.sp.r1216 : g == 3 ? nd.sp.r32
+                                        : g == 4 ? nd.sp.rcag_e : nd.sp.roag_e;
+                            sEff[e]  = r;
+                            sDeff[e] = dr;
+                            if (g == 0 || g >= 4)
+                            {
+                                const int l = g == 0 ? nd.sp.r3a_l : g == 4 ? nd.sp.rcag_l : nd.sp.roag_l;
+                                sL[l]       = rl;
+                                sDl[l]      = drl;
+                            }
+                        };
+                        special_combine(Tg, p, q, v1, v2, store);
                     }
-                    else if (g == 1)
+                }
+                else
+                {
+                    const int g = special_task<NET>(tid);
+                    if (g >= 0)
                     {
-                        special_c12c12(Tg, r, dr);
-                        sEff[nd.sp.r24] = r, sDeff[nd.sp.r24] = dr;
-                    }
-                    else if (g == 2)
-                    {
-                        special_c12o16(Tg, r, dr);
-                        sEff[nd.sp.r1216] = r, sDeff[nd.sp.r1216] = dr;
-                    }
-                    else if (g == 3)
-                    {
-                        special_o16o16(Tg, r, dr);
-                        sEff[nd.sp.r32] = r, sDeff[nd.sp.r32] = dr;
-                    }
-                    else if (g == 4)
-                    {
-                        special_c12ag(Tg, r, rl, dr, drl);
-                        sEff[nd.sp.rcag_e] = r, sDeff[nd.sp.rcag_e] = dr, sL[nd.sp.rcag_l] = rl,
-                        sDl[nd.sp.rcag_l] = drl;
-                    }
-                    else
-                    {
-                        special_o16ag(Tg, r, rl, dr, drl);
-                        sEff[nd.sp.roag_e] = r, sDeff[nd.sp.roag_e] = dr, sL[nd.sp.roag_l] = rl,
-                        sDl[nd.sp.roag_l] = drl;
+                        double r = 0, rl = 0, dr = 0, drl = 0;
+                        if (g == 0)
+                        {
+                            special_triple_alpha(Tg, r, rl, dr, drl);
+                            sEff[nd.sp.r3a_e] = r, sDeff[nd.sp.r3a_e] = dr, sL[nd.sp.r3a_l] = rl, sDl[nd.sp.r3a_l] = drl;
+                        }
+                        else if (g == 1)
+                        {
+                            special_c12c12(Tg, r, dr);
+                            sEff[nd.sp.r24] = r, sDeff[nd.sp.r24] = dr;
+                        }
+                        else if (g == 2)
+                        {
+                            special_c12o16(Tg, r, dr);
+                            sEff[nd.sp.r1216] = r, sDeff[nd.sp.r1216] = dr;
+                        }
+                        else if (g == 3)
+                        {
+                            special_o16o16(Tg, r, dr);
+                            sEff[nd.sp.r32] = r, sDeff[nd.sp.r32] = dr;
+                        }
+                        else if (g == 4)
+                        {
+                            special_c12ag(Tg, r, rl, dr, drl);
+                            sEff[nd.sp.rcag_e] = r, sDeff[nd.sp.rcag_e] = dr, sL[nd.sp.rcag_l] = rl,
+                            sDl[nd.sp.rcag_l] = drl;
+                        }
+                        else
+                        {
+                            special_o16ag(Tg, r, rl, dr, drl);
+                            sEff[nd.sp.roag_e] = r, sDeff[nd.sp.roag_e] = dr, sL[nd.sp.roag_l] = rl,
+                            sDl[nd.sp.roag_l] = drl;
+                        }
                     }
                 }
                 if (NET == 87 && tid == 98) // wavefront 1: fitted rates only, the lightest

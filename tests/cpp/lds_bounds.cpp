@@ -152,6 +152,9 @@ void run()
         IN(tag + "sEos[c]", L::oEos + c, L::oEos, L::oJv);
     for (int e = 0; e < nnz; ++e)
         IN(tag + "sJv[e]", L::oJv + e, L::oJv, L::phaseEnd);
+    // the hand-fitted groups' exp / pow values, staged in sJv during the rate phase
+    for (int k = 0; k < spx::N1 + spx::NE2; ++k)
+        IN(tag + "hand-fitted value sJv[k]", L::oJv + k, L::oJv, L::oJv + C::NNZ);
     for (int r = 0; r <= N; ++r)
         IN(tag + "sX[r]", L::oX + r, L::oX, L::oRow0);
     if (MIX)
