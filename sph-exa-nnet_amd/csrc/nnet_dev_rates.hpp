@@ -571,19 +571,23 @@ __device__ __forceinline__ double coulomb_weak(double x)
     return a * x * sqrt(x) + b * pow(x, c);
 }
 
-// The electron-capture table interpolation of net87 (net87.cpp:102-141).
-__device__ __forceinline__ void electron_interpolate(const double* logT, const double* logR, const double* tab, double temp,
-                                     double rhoElec, double* rate)
+// The electron-capture table interpolation of net87 (net87.cpp:102-141), by the 64 lanes of one
+// wavefront (k = lane).  The reference finds the grid positions by linear searches (the first
+// point not below log T, resp. log(rho Ye)); both grids ascend strictly, so that position is the
+// number of grid points below the value, counted here with one ballot per grid instead of up to
+// 92 dependent loads.  Lane c < 12 then interpolates rate c with the reference's expression.
+__device__ __forceinline__ void electron_interpolate_wave(const double* logT, const double* logR, const double* tab,
+                                                          double temp, double rhoElec, double* rate, int k)
 {
-    const int nT = 41, nR = 51, nC = 12;
-    int       it = 0;
-    double    lt = log10(temp);
-    while (it < nT && logT[it] < lt)
-        ++it;
-    int    ir = 0;
-    double lr = log10(rhoElec);
-    while (ir < nR && logR[ir] < lr)
-        ++ir;
+    constexpr int nT = 41, nR = 51, nC = 12;
+    static_assert(nT <= 64 && nR <= 64 && nC <= 64, "one grid point per lane");
+    const double lt = log10(temp);
+    const double lr = log10(rhoElec);
+    const double gT = logT[k < nT ? k : 0];
+    const double gR = logR[k < nR ? k : 0];
+    int          it = (int)__popcll(__ballot(k < nT && gT < lt));
+    int          ir = (int)__popcll(__ballot(k < nR && gR < lr));
+    if (k >= nC) return;
     const int it0 = (it - 1 > 0) ? it - 1 : 0, ir0 = (ir - 1 > 0) ? ir - 1 : 0;
     it            = (it < nT - 1) ? it : nT - 1;
     ir            = (ir < nR - 1) ? ir : nR - 1;
@@ -595,10 +599,10 @@ __device__ __forceinline__ void electron_interpolate(const double* logT, const d
     x2x1              = x2x1 == 0 ? 2 : x2x1;
     double y2y1       = logR[ir] - logR[ir0];
     y2y1              = y2y1 == 0 ? 2 : y2y1;
-    for (int c = 0; c < nC; ++c)
-        rate[c] = (tab[(it0 * nR + ir0) * nC + c] * x2x * y2y + tab[(it * nR + ir0) * nC + c] * xx1 * y2y +
-                   tab[(it0 * nR + ir) * nC + c] * x2x * yy1 + tab[(it * nR + ir) * nC + c] * xx1 * yy1) /
-                  (x2x1 * y2y1);
+    const int c       = k;
+    rate[c] = (tab[(it0 * nR + ir0) * nC + c] * x2x * y2y + tab[(it * nR + ir0) * nC + c] * xx1 * y2y +
+               tab[(it0 * nR + ir) * nC + c] * x2x * yy1 + tab[(it * nR + ir) * nC + c] * xx1 * yy1) /
+              (x2x1 * y2y1);
 }
 
 } // namespace nnet_amd

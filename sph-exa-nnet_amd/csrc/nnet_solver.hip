@@ -925,8 +925,8 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
                         }
                     }
                 }
-                if (NET == 87 && tid == 98) // wavefront 1: fitted rates only, the lightest
-                    electron_interpolate(nd.elogT, nd.elogR, nd.etab, Tg, sYb[86] * rho_t, sEv);
+                if (NET == 87 && (tid >> 6) == 1) // wavefront 1: fitted rates only, the lightest
+                    electron_interpolate_wave(nd.elogT, nd.elogR, nd.etab, Tg, sYb[86] * rho_t, sEv, tid & 63);
                 if (NET != 14 && tid == 225) sEff[3] = sDeff[3] = sL[3] = sDl[3] = 0; // unused slots
             }
             psync<WAVE>();
