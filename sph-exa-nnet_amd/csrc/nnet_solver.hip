@@ -586,12 +586,12 @@ struct TileLU
 template<int NET>
 __device__ __forceinline__ int special_task(int tid)
 {
-    // net86/net87: groups 0-3 on lanes 32, 33 of wavefronts 0 and 1 (next to the fitted rates),
-    // group 4 on wavefront 2 (fitted rates + species); group 5 on wavefront 1 for net86 and on
-    // wavefront 3 (species + EOS) for net87, whose wavefront 1 runs the electron captures: the
-    // placements that measured the shortest rate phase (its slowest wavefront sets its length)
-    if (NET == 86) return (tid == 32 || tid == 33) ? tid - 32 : (tid == 96 || tid == 97) ? tid - 94 : tid == 160 ? 4 : tid == 98 ? 5 : -1;
-    if (NET == 87) return (tid == 32 || tid == 33) ? tid - 32 : (tid == 96 || tid == 97) ? tid - 94 : tid == 160 ? 4 : tid == 233 ? 5 : -1;
+    // net86/net87: groups 0, 1 and 3 on wavefront 0 and groups 2 and 5 on wavefront 1 (next to
+    // the fitted rates), group 4 on wavefront 2 (fitted rates + species); wavefront 3 runs the
+    // species and the EOS lane.  These placements measured the shortest rate phase (its slowest
+    // wavefront sets its length).
+    if (NET == 86) return (tid == 32 || tid == 33) ? tid - 32 : tid == 96 ? 2 : tid == 34 ? 3 : tid == 160 ? 4 : tid == 98 ? 5 : -1;
+    if (NET == 87) return (tid == 32 || tid == 33) ? tid - 32 : tid == 96 ? 2 : tid == 34 ? 3 : tid == 160 ? 4 : tid == 99 ? 5 : -1;
     return -1; // net14: the lane-parallel form on its one wavefront
 }
 
