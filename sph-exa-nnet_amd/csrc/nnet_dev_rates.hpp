@@ -4,8 +4,10 @@
 //     reverse from the partition-function ratio, and both T-derivatives;
 //   * the six hand-fitted rates (3-alpha, 12C+12C, 12C+16O, 16O+16O, 12C(a,g), 16O(a,g)) with
 //     reverses and derivatives: net86 / net87, one group per lane on lanes of different
-//     wavefronts (the serial forms); net14 (one wavefront per particle), their exp / pow values
-//     one per lane of the wavefront, then combined (the lane-parallel form);
+//     wavefronts (the serial forms, which take the t9 powers RatePow / DerivPow -- functions of
+//     T alone -- computed once per wavefront rather than once per group); net14 (one wavefront
+//     per particle), their exp / pow values one per lane of the wavefront, then combined (the
+//     lane-parallel form);
 //   * one lane per species computes the Coulomb-corrected binding energy and the screening
 //     chemical potential mu/kT.
 // A second phase applies screening per rate and scatters into the reaction-list order.
@@ -107,10 +109,10 @@ struct DerivPow
 };
 
 // Group 0: triple alpha, its reverse, and derivatives (net14.cpp:218-243, 366-423).
-__device__ __forceinline__ void special_triple_alpha(double T, double& r, double& rl, double& dr, double& drl)
+__device__ __forceinline__ void special_triple_alpha(double T, const RatePow& p, const DerivPow& q, double& r, double& rl,
+                                                     double& dr, double& drl)
 {
     {
-        const RatePow p(T);
         const double r2abe = (7.40e+05 * p.t9i32) * exp(-1.0663 * p.t9i) +
                              4.164e+09 * p.t9i23 * exp(-13.49 * p.t9i13 - p.t92 / 0.009604) *
                                  (1. + 0.031 * p.t913 + 8.009 * p.t923 + 1.732 * p.t9 + 49.883 * p.t943 + 27.426 * p.t953);
@@ -127,7 +129,6 @@ __device__ __forceinline__ void special_triple_alpha(double T, double& r, double
         const double rev = 2.e20 * exp(-84.419412e0 * p.t9i);
         rl               = r * rev * p.t93;
     }
-    const DerivPow q(T);
     {
         const double vA  = -24.811 * q.t9i;
         const double vB  = -1.0663 * q.t9i;
@@ -160,16 +161,14 @@ __device__ __forceinline__ void special_triple_alpha(double T, double& r, double
 }
 
 // Group 1: 12C + 12C (net14.cpp:248-259, 428-443).
-__device__ __forceinline__ void special_c12c12(double T, double& r, double& dr)
+__device__ __forceinline__ void special_c12c12(double T, const RatePow& p, const DerivPow& q, double& r, double& dr)
 {
     {
-        const RatePow p(T);
         const double  t9a   = p.t9 / (1. + 0.0396 * p.t9);
         const double  t9a13 = pow(t9a, 1. / 3.);
         const double  t9a56 = pow(t9a, 5. / 6.);
         r                   = 4.27e+26 * t9a56 * p.t9i32 * exp(-84.165 / t9a13 - 2.12e-03 * p.t93);
     }
-    const DerivPow q(T);
     const double   vA   = q.t9 / (1. + .0396 * q.t9);
     const double   vA56 = pow(vA, 5. / 6.);
     const double   vB   = -84.165 * pow(vA, -1. / 3.) - 2.12e-3 * q.t93;
@@ -179,13 +178,12 @@ __device__ __forceinline__ void special_c12c12(double T, double& r, double& dr)
 }
 
 // Group 2: 12C + 16O (net14.cpp:264-281, 448-473).
-__device__ __forceinline__ void special_c12o16(double T, double& r, double& dr)
+__device__ __forceinline__ void special_c12o16(double T, const RatePow& p, const DerivPow& q, double& r, double& dr)
 {
     r  = 0;
     dr = 0;
     if (T >= 5e8)
     {
-        const RatePow p(T);
         const double  t9ap    = p.t9 / (1. + 0.055 * p.t9);
         const double  t9a2p   = t9ap * t9ap;
         const double  t9a13p  = pow(t9ap, 1. / 3.);
@@ -196,7 +194,6 @@ __device__ __forceinline__ void special_c12o16(double T, double& r, double& dr)
     const double t9 = T * 1.0e-09;
     if (t9 > .5)
     {
-        const DerivPow q(T);
         const double   vA   = q.t9 / (1. + .055 * q.t9);
         const double   vA56 = pow(vA, 5. / 6.);
         const double   vB   = -106.594 * pow(vA, -1. / 3.);
@@ -215,30 +212,27 @@ __device__ __forceinline__ void special_c12o16(double T, double& r, double& dr)
 }
 
 // Group 3: 16O + 16O (net14.cpp:286-293, 478-487).
-__device__ __forceinline__ void special_o16o16(double T, double& r, double& dr)
+__device__ __forceinline__ void special_o16o16(double T, const RatePow& p, const DerivPow& q, double& r, double& dr)
 {
     {
-        const RatePow p(T);
         r = 7.10e+36 * p.t9i23 * exp(-135.93 * p.t9i13 - 0.629 * p.t923 - 0.445 * p.t943 + 0.0103 * p.t92);
     }
-    const DerivPow q(T);
     const double   vA  = -135.93e0 * q.t9i13 - .629 * q.t923 - .445 * q.t943 + .0103 * q.t92;
     const double   dvA = 45.31 * q.t9i43 - .629 * q.t9i13 * 2. / 3. - .445 * q.t913 * 4. / 3. + .0206 * q.t9;
     dr                 = 7.10e36 * exp(vA) * q.t9i23 * (-q.t9i * 2. / 3. + dvA) * 1.e-9;
 }
 
 // Group 4: 12C(a,g)16O, reverse, derivatives (net14.cpp:298-314, 491-538).
-__device__ __forceinline__ void special_c12ag(double T, double& r, double& rl, double& dr, double& drl)
+__device__ __forceinline__ void special_c12ag(double T, const RatePow& p, const DerivPow& q, double& r, double& rl, double& dr,
+                                              double& drl)
 {
     {
-        const RatePow p(T);
         const double  b1 = 1. + 0.0489 * p.t9i23, b2 = 1. + 0.2654 * p.t9i23;
         r = 1.04e+08 / (p.t92 * (b1 * b1)) * exp(-32.120 * p.t9i13 - p.t92 / 12.222016) +
             1.76e+08 / (p.t92 * (b2 * b2)) * exp(-32.120 * p.t9i13) + 1.25e+03 * p.t9i32 * exp(-27.499 * p.t9i) +
             1.43e-02 * p.t95 * exp(-15.541 * p.t9i);
         rl = r * 5.13e+10 * p.t9r32 * exp(-83.108047 * p.t9rm1);
     }
-    const DerivPow q(T);
     {
         const double b1  = 1. + .0489 * q.t9i23;
         const double vB  = b1 * b1;
@@ -268,15 +262,14 @@ __device__ __forceinline__ void special_c12ag(double T, double& r, double& rl, d
 }
 
 // Group 5: 16O(a,g)20Ne, reverse, derivatives (net14.cpp:319-334, 519-566).
-__device__ __forceinline__ void special_o16ag(double T, double& r, double& rl, double& dr, double& drl)
+__device__ __forceinline__ void special_o16ag(double T, const RatePow& p, const DerivPow& q, double& r, double& rl, double& dr,
+                                              double& drl)
 {
     {
-        const RatePow p(T);
         r  = (9.37e+09 * p.t9i23 * exp(-39.757 * p.t9i13 - p.t92 / 2.515396) + 62.1 * p.t9i32 * exp(-10.297 * p.t9i) +
              538. * p.t9i32 * exp(-12.226 * p.t9i) + 13. * p.t92 * exp(-20.093 * p.t9i));
         rl = r * 5.65e+10 * p.t9r32 * exp(-54.93807 * p.t9rm1);
     }
-    const DerivPow q(T);
     {
         const double s1  = q.t9 / 1.586;
         const double vB  = -39.757 * q.t9i13 - s1 * s1;

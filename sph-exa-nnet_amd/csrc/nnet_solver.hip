@@ -887,41 +887,49 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
                 }
                 else
                 {
-                    const int g = special_task<NET>(tid);
-                    if (g >= 0)
+                    // the t9 powers (functions of T alone) once per wavefront that holds a group,
+                    // in wave-uniform code, instead of once per group on its own lane
+                    static_assert(C::NT == 256, "groups on wavefronts 0-2 of four");
+                    if ((tid >> 6) < 3)
                     {
-                        double r = 0, rl = 0, dr = 0, drl = 0;
-                        if (g == 0)
+                        const RatePow  p(Tg);
+                        const DerivPow q(Tg);
+                        const int g = special_task<NET>(tid);
+                        if (g >= 0)
                         {
-                            special_triple_alpha(Tg, r, rl, dr, drl);
-                            sEff[nd.sp.r3a_e] = r, sDeff[nd.sp.r3a_e] = dr, sL[nd.sp.r3a_l] = rl, sDl[nd.sp.r3a_l] = drl;
-                        }
-                        else if (g == 1)
-                        {
-                            special_c12c12(Tg, r, dr);
-                            sEff[nd.sp.r24] = r, sDeff[nd.sp.r24] = dr;
-                        }
-                        else if (g == 2)
-                        {
-                            special_c12o16(Tg, r, dr);
-                            sEff[nd.sp.r1216] = r, sDeff[nd.sp.r1216] = dr;
-                        }
-                        else if (g == 3)
-                        {
-                            special_o16o16(Tg, r, dr);
-                            sEff[nd.sp.r32] = r, sDeff[nd.sp.r32] = dr;
-                        }
-                        else if (g == 4)
-                        {
-                            special_c12ag(Tg, r, rl, dr, drl);
-                            sEff[nd.sp.rcag_e] = r, sDeff[nd.sp.rcag_e] = dr, sL[nd.sp.rcag_l] = rl,
-                            sDl[nd.sp.rcag_l] = drl;
-                        }
-                        else
-                        {
-                            special_o16ag(Tg, r, rl, dr, drl);
-                            sEff[nd.sp.roag_e] = r, sDeff[nd.sp.roag_e] = dr, sL[nd.sp.roag_l] = rl,
-                            sDl[nd.sp.roag_l] = drl;
+                            double r = 0, rl = 0, dr = 0, drl = 0;
+                            if (g == 0)
+                            {
+                                special_triple_alpha(Tg, p, q, r, rl, dr, drl);
+                                sEff[nd.sp.r3a_e] = r, sDeff[nd.sp.r3a_e] = dr, sL[nd.sp.r3a_l] = rl, sDl[nd.sp.r3a_l] = drl;
+                            }
+                            else if (g == 1)
+                            {
+                                special_c12c12(Tg, p, q, r, dr);
+                                sEff[nd.sp.r24] = r, sDeff[nd.sp.r24] = dr;
+                            }
+                            else if (g == 2)
+                            {
+                                special_c12o16(Tg, p, q, r, dr);
+                                sEff[nd.sp.r1216] = r, sDeff[nd.sp.r1216] = dr;
+                            }
+                            else if (g == 3)
+                            {
+                                special_o16o16(Tg, p, q, r, dr);
+                                sEff[nd.sp.r32] = r, sDeff[nd.sp.r32] = dr;
+                            }
+                            else if (g == 4)
+                            {
+                                special_c12ag(Tg, p, q, r, rl, dr, drl);
+                                sEff[nd.sp.rcag_e] = r, sDeff[nd.sp.rcag_e] = dr, sL[nd.sp.rcag_l] = rl,
+                                sDl[nd.sp.rcag_l] = drl;
+                            }
+                            else
+                            {
+                                special_o16ag(Tg, p, q, r, rl, dr, drl);
+                                sEff[nd.sp.roag_e] = r, sDeff[nd.sp.roag_e] = dr, sL[nd.sp.roag_l] = rl,
+                                sDl[nd.sp.roag_l] = drl;
+                            }
                         }
                     }
                 }
