@@ -188,7 +188,8 @@ struct AsmImage
     std::vector<uint32_t> img;     // the image, 32-bit words
     std::vector<uint32_t> lanemap; // [lane][4]: register-slot mask (lo, hi), first non-zero id, 0
     int                   nnz = 0;
-    int oRx = 0, oEnt16 = 0, oSpPtr16 = 0, oSpEnt = 0, oNzPtr16 = 0, oNzEnt = 0, oRowPtr16 = 0, oRowEnt = 0, oJord16 = 0;
+    int oRx = 0, oEnt16 = 0, oSpPtr16 = 0, oSpEnt = 0, oNzPtr16 = 0, oNzEnt = 0, oRowPtr16 = 0, oRowEnt = 0, oJord16 = 0,
+        oMaPerm16 = 0;
 };
 
 // Builds the image of a validated reaction list (reaction r: reactants [rb[r], pb[r]) and
@@ -329,6 +330,44 @@ inline std::string build_asm_image(int kind, int d, int nR, const int* rb, const
         for (int e = 0; e < nnz; ++e)
             h.push_back((uint16_t)e);
         std::stable_sort(h.begin(), h.end(), [&](uint16_t a, uint16_t b) { return cost[a] > cost[b]; });
+        push_half(h);
+    }
+    // mass-action order: slot s of the per-reaction factor phase (lane s % NT, pass s / NT)
+    // computes reaction perm[s].  A wavefront runs as many passes as its busiest lane, so the
+    // wavefronts with the most passes take the cheapest reactions (fewest reactant entries, no
+    // count above one: one load chain, no division) in all their passes; with a single pass
+    // everywhere the order is the identity.
+    out.oMaPerm16 = (int)img.size() * 2;
+    {
+        const int        NTk = kind == 14 ? Cfg<14>::NT : Cfg<86>::NT;
+        std::vector<int> perm(nR);
+        for (int r = 0; r < nR; ++r)
+            perm[r] = r;
+        if (nR > NTk)
+        {
+            std::vector<int> rcost(nR), order(nR), slots(nR), passes((NTk + 63) / 64, 0);
+            for (int r = 0; r < nR; ++r)
+            {
+                int c2 = 0;
+                for (int k = rb[r]; k < pb[r]; ++k)
+                    c2 += nums[k] > 1;
+                rcost[r] = (pb[r] - rb[r]) + 2 * c2;
+                order[r] = r;
+            }
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rcost[a] < rcost[b]; });
+            for (int s2 = 0; s2 < nR; ++s2)
+            {
+                passes[(s2 % NTk) / 64] = s2 / NTk + 1;
+                slots[s2]               = s2;
+            }
+            std::stable_sort(slots.begin(), slots.end(),
+                             [&](int a, int b) { return passes[(a % NTk) / 64] > passes[(b % NTk) / 64]; });
+            for (int i = 0; i < nR; ++i)
+                perm[slots[i]] = order[i];
+        }
+        h.clear();
+        for (int r = 0; r < nR; ++r)
+            h.push_back((uint16_t)perm[r]);
         push_half(h);
     }
 

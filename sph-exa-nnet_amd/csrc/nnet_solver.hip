@@ -94,7 +94,7 @@ struct NetDev
     //   row_ptr16[d+1], nz_col8[nnz]: non-zeros of row a are [row_ptr[a], row_ptr[a+1]), columns
     //                 ascending
     const uint32_t* img;
-    int             img_words, oRx, oEnt16, oSpPtr16, oSpEnt, oNzPtr16, oNzEnt, oRowPtr16, oRowEnt, oJord16;
+    int             img_words, oRx, oEnt16, oSpPtr16, oSpEnt, oNzPtr16, oNzEnt, oRowPtr16, oRowEnt, oJord16, oMaPerm16;
     // per lane, per register slot: index of the Jacobian non-zero the slot holds (-1: zero),
     // laid out [slot][lane] for coalesced loads
     const uint4*   lanemap; // [lane]: register-slot mask (x, y), first non-zero id (z)
@@ -650,6 +650,7 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
     const uint16_t* nzp16  = reinterpret_cast<const uint16_t*>(simg) + nd.oNzPtr16;
     const uint32_t* nzent  = simg + nd.oNzEnt;
     const uint16_t* jord16 = reinterpret_cast<const uint16_t*>(simg) + nd.oJord16;
+    const uint16_t* maperm = reinterpret_cast<const uint16_t*>(simg) + nd.oMaPerm16;
     const uint16_t* rowp16 = reinterpret_cast<const uint16_t*>(simg) + nd.oRowPtr16;
     const uint32_t* rowent = simg + nd.oRowEnt;
 
@@ -991,8 +992,11 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
             psync<WAVE>();
 
             // ---- per-reaction mass-action factors (nuclear_net.hpp:355-371, 404-426) ----
-            for (int r = tid; r < nd.nR; r += NT)
+            // slot s computes reaction maperm[s]: the wavefronts that run a second pass take the
+            // cheapest reactions
+            for (int s = tid; s < nd.nR; s += NT)
             {
+                const int    r = maperm[s];
                 double       v = sRate[r], dv = sDrt[r], jr = sRate[r];
                 const int    q0 = rx[r] & 0xffff, q1 = rx[r] >> 16;
                 for (int q = q0; q < q1; ++q)
@@ -1933,7 +1937,7 @@ static int build_network(nnet_ctx* ctx, nnet_network* nw)
     n.etab = (double*)(B + oElTab), n.BEe = NNET87_ELECTRON_BE_MEV[0] * NNET_MEV_TO_ERG;
     n.img = (uint32_t*)(B + oImg), n.img_words = (int)img.size(), n.oRx = im.oRx, n.oEnt16 = im.oEnt16;
     n.oSpPtr16 = im.oSpPtr16, n.oSpEnt = im.oSpEnt, n.oNzPtr16 = im.oNzPtr16, n.oNzEnt = im.oNzEnt;
-    n.oRowPtr16 = im.oRowPtr16, n.oRowEnt = im.oRowEnt, n.oJord16 = im.oJord16, n.lanemap = (const uint4*)(B + oLane);
+    n.oRowPtr16 = im.oRowPtr16, n.oRowEnt = im.oRowEnt, n.oJord16 = im.oJord16, n.oMaPerm16 = im.oMaPerm16, n.lanemap = (const uint4*)(B + oLane);
     HIP_TRY(ctx, hipMemcpy(B + oSelf, &n, sizeof(NetDev), hipMemcpyHostToDevice));
     nw->d_dev = (const NetDev*)(B + oSelf);
     return NNET_OK;

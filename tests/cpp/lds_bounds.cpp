@@ -101,6 +101,19 @@ void run()
         IN(tag + "jord16", im.oJord16 + k, 0, 2 * W);
         IN(tag + "jord16 non-zero", h16[im.oJord16 + k], 0, nnz);
     }
+    // mass-action order: a permutation of the reactions (each computed by exactly one slot)
+    {
+        std::vector<int> seen(nR, 0);
+        for (int k = 0; k < nR; ++k)
+        {
+            IN(tag + "maperm16", im.oMaPerm16 + k, 0, 2 * W);
+            const int r = h16[im.oMaPerm16 + k];
+            IN(tag + "maperm16 reaction", r, 0, nR);
+            if (r >= 0 && r < nR) ++seen[r];
+        }
+        for (int r = 0; r < nR; ++r)
+            check(seen[r] == 1, tag + "maperm16 reaction computed once", seen[r], 1, 2);
+    }
     (void)E;
 
     // ---- the LDS arrays of the kernel (offsets in doubles) ----
