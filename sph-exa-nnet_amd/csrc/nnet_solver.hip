@@ -1077,48 +1077,68 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
                 }
                 sJv[e] = J;
             }
+            // dY/dt and d(dY/dt)/dT per species: signed-count sums in list order.  Nothing reads
+            // them before the matrix fill, so with several wavefronts per particle each sum takes
+            // its first KPRE entries here and the rest after the barrier, beside the RHS
+            // correction of the other lanes (the longest species lists -- 140 entries for two
+            // net86 species -- would otherwise hold every wavefront at the barrier); one
+            // wavefront has nothing to overlap.  The additions keep their order across the split.
+            constexpr bool SPLIT = C::NT > 64;
+            constexpr int  KPRE  = SPLIT ? 72 : 1 << 20; // 72 measured faster than 96 on net86
+            static_assert(2 * D <= NT, "species lanes [NT - D, NT) and RHS lanes [0, D) are disjoint");
+            const int     sps  = tid - (NT - D);
+            double        spa = 0., spb = 0.;
+            int           spe = 0, spe1 = 0;
+            auto species_sum = [&](int e1) __attribute__((always_inline))
             {
-                // dY/dt and d(dY/dt)/dT per species: signed-count sums in list order
-                const int s = tid - (NT - D);
-                if (s >= 0)
+                int e = spe;
+                for (; e + 8 <= e1; e += 8)
                 {
-                    double a = 0., b = 0.;
-                    int    e = spp16[s];
-                    const int e1 = spp16[s + 1];
-                    for (; e + 8 <= e1; e += 8)
+                    uint32_t w[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        w[u] = spent[e + u];
+                    __builtin_amdgcn_sched_barrier(0);
+                    double x[8], y[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
                     {
-                        uint32_t w[8];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u)
-                            w[u] = spent[e + u];
-                        __builtin_amdgcn_sched_barrier(0);
-                        double x[8], y[8];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u)
-                        {
-                            const int r = w[u] & 0xffff;
-                            x[u]        = sVal[r];
-                            y[u]        = sDval[r];
-                        }
-                        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                        for (int u = 0; u < 8; ++u)
-                        {
-                            const double c = (double)(int8_t)(w[u] >> 16);
-                            a              = a + x[u] * c;
-                            b              = b + y[u] * c;
-                        }
+                        const int r = w[u] & 0xffff;
+                        x[u]        = sVal[r];
+                        y[u]        = sDval[r];
                     }
-                    for (; e < e1; ++e)
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
                     {
-                        const uint32_t w = spent[e];
-                        const int      r = w & 0xffff;
-                        const double   c = (double)(int8_t)(w >> 16);
-                        a                = a + sVal[r] * c;
-                        b                = b + sDval[r] * c;
+                        const double c = (double)(int8_t)(w[u] >> 16);
+                        spa            = spa + x[u] * c;
+                        spb            = spb + y[u] * c;
                     }
-                    sDYd[s] = a;
-                    sDYT[s] = b;
+                }
+                for (; e < e1; ++e)
+                {
+                    const uint32_t w = spent[e];
+                    const int      r = w & 0xffff;
+                    const double   c = (double)(int8_t)(w >> 16);
+                    spa              = spa + sVal[r] * c;
+                    spb              = spb + sDval[r] * c;
+                }
+                spe = e;
+            };
+            // Yg - Y per species for the RHS correction, into the screening array (dead after the
+            // rate phase): the difference the reference forms inside that sum, formed once here
+            double* sDYg = sMu;
+            if (sps >= 0)
+            {
+                sDYg[sps] = sYb[sps] - sY[sps];
+                spe  = spp16[sps];
+                spe1 = spp16[sps + 1];
+                species_sum(spe1 - spe > KPRE ? spe + KPRE : spe1);
+                if constexpr (!SPLIT)
+                {
+                    sDYd[sps] = spa;
+                    sDYT[sps] = spb;
                 }
             }
             // this lane's register-slot mask and first non-zero id (for the matrix fill)
@@ -1137,27 +1157,30 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
                     for (int u = 0; u < 8; ++u)
                         w[u] = rowent[e + u];
                     __builtin_amdgcn_sched_barrier(0);
-                    double jv[8], yb[8], y0[8];
+                    double jv[8], dy[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
                     {
-                        const int b = w[u] >> 16;
-                        jv[u]       = sJv[w[u] & 0xffff];
-                        yb[u]       = sYb[b];
-                        y0[u]       = sY[b];
+                        jv[u] = sJv[w[u] & 0xffff];
+                        dy[u] = sDYg[w[u] >> 16];
                     }
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
-                        corr = corr + jv[u] * (yb[u] - y0[u]);
+                        corr = corr + jv[u] * dy[u];
                 }
                 for (; e < e1; ++e)
                 {
                     const uint32_t w = rowent[e];
-                    const int      b = w >> 16;
-                    corr             = corr + sJv[w & 0xffff] * (sYb[b] - sY[b]);
+                    corr             = corr + sJv[w & 0xffff] * sDYg[w >> 16];
                 }
                 sCorr[tid] = corr;
+            }
+            else if (SPLIT && sps >= 0)
+            {
+                species_sum(spe1);
+                sDYd[sps] = spa;
+                sDYT[sps] = spb;
             }
             psync<WAVE>();
 
