@@ -89,8 +89,15 @@ struct LuIdx
     static constexpr int N = C::N, TT = C::TT, WR = C::WR, TI = TT / WR, NPAD = 16 * TT;
     static constexpr int NS = (N + 3) / 4; // blocks of four pivots
 
-    // first packed entry of normalised row k (columns k+1 .. N) / of multiplier row k (0 .. k-1)
-    static constexpr int urow(int k) { return k * N - k * (k - 1) / 2 - k - 1; }
+    // normalised upper entry (k, j), k < j <= N: net86 / net87 pack it by columns (column j holds
+    // rows 0 .. j-1, so the back substitution's lanes read consecutive words: measured faster),
+    // net14 by rows (row k holds columns k+1 .. N: its single wavefront's LU stores are then the
+    // consecutive ones, measured faster there); first packed entry of multiplier row k (columns
+    // 0 .. k-1)
+    static constexpr int uidx(int k, int j)
+    {
+        return NET == 14 ? k * N - k * (k - 1) / 2 - k - 1 + j : j * (j - 1) / 2 + k;
+    }
     static constexpr int lrow(int k) { return k * (k - 1) / 2; }
     // panel buffers of block `step`: generation (step & 1) of [row][4] panel columns P, followed
     // by [column][4] pivot rows RT

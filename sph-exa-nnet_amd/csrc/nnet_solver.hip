@@ -275,7 +275,7 @@ struct TileLU
 
     using X = LuIdx<NET>;
     static_assert(X::NS == NS && X::NPAD == NPAD, "index functions of the same configuration");
-    static __device__ __forceinline__ int urow(int k) { return X::urow(k); }
+    static __device__ __forceinline__ int uidx(int k, int j) { return X::uidx(k, j); }
     static __device__ __forceinline__ int lrow(int k) { return X::lrow(k); }
 
     static __device__ __forceinline__ R pick4(const R (&x)[4], int q)
@@ -360,7 +360,7 @@ struct TileLU
             {
 #pragma unroll
                 for (int c = q + 1; c < 4; ++c)
-                    if (p0 + c <= N) sU[urow(p0 + q) + p0 + c] = Un[q][c];
+                    if (p0 + c <= N) sU[uidx(p0 + q, p0 + c)] = Un[q][c];
                 if constexpr (STORE_L)
                 {
                     sRinv[p0 + q] = rinv[q];
@@ -421,7 +421,7 @@ struct TileLU
                 for (int q = 0; q < 4; ++q)
                 {
                     RT(j, q) = q < nq ? u[q] : R(0);
-                    if (q < nq) sU[urow(p0 + q) + j] = u[q];
+                    if (q < nq) sU[uidx(p0 + q, j)] = u[q];
                 }
             }
         }
@@ -508,10 +508,10 @@ struct TileLU
                 {
                     const int j = j0 - s, row = lane + 64 * a;
                     // the index is selected before the load, so the address stays inside the
-                    // packed rows also for lanes the condition excludes (urow(row) + j is negative
-                    // for row = 0, j <= 0, and for rows beyond 2N)
+                    // packed factor also for lanes the condition excludes (rows at or below the
+                    // diagonal, j <= 0)
                     const bool ok = j > 0 && row < j;
-                    const R    x  = sU[ok ? urow(row) + j : 0];
+                    const R    x  = sU[ok ? uidx(row, j) : 0];
                     u[s][a]       = ok ? x : R(0);
                 }
             __builtin_amdgcn_sched_barrier(0);
@@ -1298,7 +1298,7 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
                     for (int a = 0; a < NA; ++a)
                     {
                         const int row = tid + 64 * a;
-                        const double x = sU[row < N ? TileLU<NET, double, false, WAVE>::urow(row) + N : 0];
+                        const double x = sU[row < N ? TileLU<NET, double, false, WAVE>::uidx(row, N) : 0];
                         acc[a]         = row < N ? x : 0.;
                     }
                     TileLU<NET, double, false, WAVE>::back(sU, acc, tid);
@@ -1336,7 +1336,7 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
                     for (int a = 0; a < NA; ++a)
                     {
                         const int row = tid + 64 * a;
-                        const float x = sU32[row < N ? LU32::urow(row) + N : 0];
+                        const float x = sU32[row < N ? LU32::uidx(row, N) : 0];
                         acc[a]        = row < N ? x : 0.f;
                     }
                     LU32::back(sU32, acc, tid);

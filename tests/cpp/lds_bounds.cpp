@@ -282,8 +282,8 @@ void run()
                         for (int c = q + 1; c < 4; ++c)
                             if (p0 + c <= N)
                             {
-                                IN(t2 + "sU diag", X::urow(p0 + q) + p0 + c, 0, uEnd);
-                                storeU(X::urow(p0 + q) + p0 + c);
+                                IN(t2 + "sU diag", X::uidx(p0 + q, p0 + c), 0, uEnd);
+                                storeU(X::uidx(p0 + q, p0 + c));
                             }
                         if (fp32)
                         {
@@ -319,8 +319,8 @@ void run()
                             IN(t2 + "column lane RT", X::prow(step, j, q), 0, panEnd);
                             if (q < nq)
                             {
-                                IN(t2 + "column lane sU", X::urow(p0 + q) + j, 0, uEnd);
-                                storeU(X::urow(p0 + q) + j);
+                                IN(t2 + "column lane sU", X::uidx(p0 + q, j), 0, uEnd);
+                                storeU(X::uidx(p0 + q, j));
                             }
                         }
                 }
@@ -345,7 +345,7 @@ void run()
         // is stored by exactly one lane of exactly one block
         for (int kk = 0; kk < N; ++kk)
             for (int j = kk + 1; j <= N; ++j)
-                check(uStores[X::urow(kk) + j] == 1, t2 + "sU entry stored once", X::urow(kk) + j, 1, 2);
+                check(uStores[X::uidx(kk, j)] == 1, t2 + "sU entry stored once", X::uidx(kk, j), 1, 2);
         if (fp32)
             for (int i = 1; i < N; ++i)
                 for (int c = 0; c < i; ++c)
@@ -356,7 +356,7 @@ void run()
             for (int a = 0; a < NA; ++a)
             {
                 const int row = lane + 64 * a;
-                IN(t2 + "rhs sU[urow(row)+N]", row < N ? X::urow(row) + N : 0, 0, uEnd);
+                IN(t2 + "rhs sU[uidx(row, N)]", row < N ? X::uidx(row, N) : 0, 0, uEnd);
             }
             for (int j0 = N - 1; j0 > 0; j0 -= 4)
                 for (int s = 0; s < 4; ++s)
@@ -364,7 +364,7 @@ void run()
                     {
                         const int j = j0 - s, row = lane + 64 * a;
                         const bool ok = j > 0 && row < j;
-                        IN(t2 + "back sU", ok ? X::urow(row) + j : 0, 0, uEnd);
+                        IN(t2 + "back sU", ok ? X::uidx(row, j) : 0, 0, uEnd);
                     }
             if (fp32)
                 for (int j0 = 0; j0 < N; j0 += 4)
