@@ -445,8 +445,8 @@ struct TileLU
         // The tiles holding the next block's panel (its tile row and tile column) are updated
         // first and published; the other tiles' MFMAs are issued after the publish, so they
         // execute while the wave waits at the barrier and factors the next diagonal block (each
-        // tile still receives its updates in block order).  C3 step 2.35 s against 2.39 s with
-        // the publish after all updates.
+        // tile still receives its updates in block order).  A net86 Helmholtz step of 2,097,152
+        // particles took 2.35 s against 2.39 s with the publish after all updates.
         constexpr int Tn = (4 * (STEP + 1)) >> 4;
         auto update = [&](bool panel_tiles) __attribute__((always_inline))
         {
@@ -2140,7 +2140,7 @@ static bool capturing(hipStream_t st)
 }
 
 // Checks a call makes before it touches the stream when the stream is being captured.  A
-// captured multi-batch call with a mixed-precision batch (the C5 hydro loop) ended in a host
+// captured multi-batch call with a mixed-precision batch (a net14 + net86 + net87 hydro loop) ended in a host
 // segmentation fault whose cause has not been shown; until it is, that combination is refused
 // rather than run (the FP64 multi-batch capture is tested and stays enabled).
 static int check_capture(nnet_ctx* c, const nnet_batch* batches, int nb, int n_live)
