@@ -1032,10 +1032,16 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
 
             // ---- Jacobian non-zeros (firstOrderDYFromReactions) and species sums
             //      (derivativesFromReactions), each summed in the reference's order ----
-            for (int k = tid; k < nd.nnz; k += NT)
+            // non-zeros are handed out in descending order of their list lengths, so the lanes of
+            // one pass over the workgroup have lists of similar length; with four wavefronts the
+            // non-zeros beyond three full passes go to wavefront 1 (no species lanes, the lightest
+            // load of this phase) rather than to wavefront 0, which holds the longest lists
+            for (int pass = 0;; ++pass)
             {
-                // non-zeros are handed out in descending order of their list lengths, so the
-                // lanes of one pass over the workgroup have lists of similar length
+                int k = tid + NT * pass;
+                if constexpr (NT == 256)
+                    if (pass >= 3) k = (tid >> 6) == 1 ? 3 * NT + 64 * (pass - 3) + (tid - 64) : nd.nnz;
+                if (k >= nd.nnz) break;
                 const int e = jord16[k];
                 // J = sum over the entry's contributions in list order of count * mass-action
                 // derivative, skipping reactions whose derivative factor is <= 1e-40; eight
