@@ -587,9 +587,9 @@ template<int NET>
 __device__ __forceinline__ int special_task(int tid)
 {
     // net86/net87: groups 0, 1 and 3 on wavefront 0 and groups 2 and 5 on wavefront 1 (next to
-    // the fitted rates), group 4 on wavefront 2 (fitted rates + species); wavefront 3 runs the
-    // species and the EOS lane.  These placements measured the shortest rate phase (its slowest
-    // wavefront sets its length).
+    // the fitted rates; net87's electron captures also run on wavefront 1), group 4 on wavefront 2
+    // (fitted rates + species); wavefront 3 runs the species and the EOS lane.  These placements
+    // measured the shortest rate phase (its slowest wavefront sets its length).
     if (NET == 86) return (tid == 32 || tid == 33) ? tid - 32 : tid == 96 ? 2 : tid == 34 ? 3 : tid == 160 ? 4 : tid == 98 ? 5 : -1;
     if (NET == 87) return (tid == 32 || tid == 33) ? tid - 32 : tid == 96 ? 2 : tid == 34 ? 3 : tid == 160 ? 4 : tid == 99 ? 5 : -1;
     return -1; // net14: the lane-parallel form on its one wavefront
@@ -934,7 +934,7 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
                         }
                     }
                 }
-                if (NET == 87 && (tid >> 6) == 1) // wavefront 1: fitted rates only, the lightest
+                if (NET == 87 && (tid >> 6) == 1) // wavefront 1 (fitted rates, groups 2 and 5)
                     electron_interpolate_wave(nd.elogT, nd.elogR, nd.etab, Tg, sYb[86] * rho_t, sEv, tid & 63);
                 if (NET != 14 && tid == 225) sEff[3] = sDeff[3] = sL[3] = sDl[3] = 0; // unused slots
             }
