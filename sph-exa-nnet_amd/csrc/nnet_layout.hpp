@@ -75,6 +75,17 @@ struct Lds
                          total = oPan + 16 * NPAD;
 };
 
+// Jacobian non-zero slot (position in the image's descending-length order) that lane `tid` of a
+// particle's NT lanes sums in pass `pass`, or a value >= nnz when it has none: pass p takes slot
+// tid + NT p; with four wavefronts the slots beyond three full passes go to wavefront 1 (no species
+// lanes, the lightest load of that phase) rather than to wavefront 0, which holds the longest lists.
+constexpr int jac_slot(int tid, int pass, int NT, int nnz)
+{
+    int k = tid + NT * pass;
+    if (NT == 256 && pass >= 3) k = (tid >> 6) == 1 ? 3 * NT + 64 * (pass - 3) + (tid - 64) : nnz;
+    return k;
+}
+
 // Matrix row held by register v of lane (lr = (l & 63) >> 4, lc = l & 15) in tile row I of the
 // register-resident Newton matrix, and its column in tile column J (the v_mfma_f64_16x16x4f64
 // result layout).

@@ -1033,14 +1033,10 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
             // ---- Jacobian non-zeros (firstOrderDYFromReactions) and species sums
             //      (derivativesFromReactions), each summed in the reference's order ----
             // non-zeros are handed out in descending order of their list lengths, so the lanes of
-            // one pass over the workgroup have lists of similar length; with four wavefronts the
-            // non-zeros beyond three full passes go to wavefront 1 (no species lanes, the lightest
-            // load of this phase) rather than to wavefront 0, which holds the longest lists
+            // one pass over the workgroup have lists of similar length (lane-to-slot map: jac_slot)
             for (int pass = 0;; ++pass)
             {
-                int k = tid + NT * pass;
-                if constexpr (NT == 256)
-                    if (pass >= 3) k = (tid >> 6) == 1 ? 3 * NT + 64 * (pass - 3) + (tid - 64) : nd.nnz;
+                const int k = jac_slot(tid, pass, NT, nd.nnz);
                 if (k >= nd.nnz) break;
                 const int e = jord16[k];
                 // J = sum over the entry's contributions in list order of count * mass-action

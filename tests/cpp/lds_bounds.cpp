@@ -101,6 +101,22 @@ void run()
         IN(tag + "jord16", im.oJord16 + k, 0, 2 * W);
         IN(tag + "jord16 non-zero", h16[im.oJord16 + k], 0, nnz);
     }
+    // Jacobian phase: every non-zero slot is summed by exactly one (lane, pass) of the kernel's loop
+    {
+        std::vector<int> seen(nnz, 0);
+        for (int tid = 0; tid < NT; ++tid)
+            for (int pass = 0;; ++pass)
+            {
+                const int k = jac_slot(tid, pass, NT, nnz);
+                if (k >= nnz) break;
+                IN(tag + "Jacobian slot", k, 0, nnz);
+                if (k >= 0) ++seen[k];
+                check(pass < 64, tag + "Jacobian passes end", pass, 0, 64);
+                if (pass >= 64) break;
+            }
+        for (int k = 0; k < nnz; ++k)
+            check(seen[k] == 1, tag + "Jacobian slot summed once", seen[k], 1, 2);
+    }
     // mass-action order: a permutation of the reactions (each computed by exactly one slot)
     {
         std::vector<int> seen(nR, 0);
