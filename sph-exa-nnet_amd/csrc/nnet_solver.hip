@@ -1289,7 +1289,9 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
             {
                 dx4 Mt[TI][TI];
                 fill(Mt);
-                psync<WAVE>(); // the packed rows below alias the assembly arrays
+                // the packed rows the LU writes alias the arrays the fill reads: the barrier after
+                // the LU's first panel publish (which writes only the panel buffers) orders every
+                // lane's fill before the first packed-row store
                 // ---- LU (eigen.hpp:179-206), four pivots per block ----
                 TileLU<NET, double, false, WAVE>::factor(Mt, sPan, sU, nullptr, nullptr, wr, wc, lr, lc, tid);
                 // ---- back substitution (eigen.hpp:208-221) ----
@@ -1328,7 +1330,8 @@ __device__ __forceinline__ void solve_particles(const NetDev& nd, const EosT& ed
                 {
                     fx4 Mt[TI][TI];
                     fill(Mt);
-                    psync<WAVE>(); // FP32 factors alias the head arrays only
+                    // FP32 factors alias the head arrays only; the first is stored after the LU's
+                    // first panel barrier
                     LU32::factor(Mt, sPan32, sU32, sL32, sRi32, wr, wc, lr, lc, tid);
                 }
                 if (tid < 64)
