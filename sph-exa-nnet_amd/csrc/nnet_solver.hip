@@ -369,6 +369,11 @@ struct TileLU
                         sL[lrow(p0 + q) + p0 + c] = Wt[q][c];
                 }
             }
+        // the last block has no trailing rows; net86 and net14 have no trailing columns there
+        // either, so the diagonal block's stores above are all it leaves (wavefront 0 runs the
+        // triangular solves after the LU and reads them in program order)
+        constexpr bool ROWS = p0 + 4 < N, COLS = p0 + 4 <= N;
+        if constexpr (!COLS) return;
 
         if (tid < HALF)
         {
@@ -426,6 +431,9 @@ struct TileLU
             }
         }
         psync<WAVE>();
+        // no trailing rows (net87's last block: only the right-hand-side column, normalised by the
+        // column lanes above and ordered before the triangular solves by that barrier): no update
+        if constexpr (!ROWS) return;
 
         // rank-4 trailing update, one MFMA per tile that holds trailing rows and columns
         // (A operand: row arow(l&15) of the tile, pivot l>>4; B operand: pivot l>>4, column l&15)
