@@ -34,11 +34,11 @@ PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix)
 # HBM bytes per particle-step of the solve kernels on the net86 C3 distribution, from rocprofv3
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE runs, scaled by the same counters on a plain 512 MiB device
 # copy (`bench.py --copy-calibration 512`: FETCH_SIZE reads 0.5x and WRITE_SIZE 1.0x of the copied
-# bytes).  FP64 (`--precision fp64 --steps 1 --warmup 0 --no-cpu`, one dispatch): 2 x 4,816,651 KiB
-# fetched + 3,970,019 KiB written.  Mixed (`--precision mixed --steps 1 --warmup 1 --no-cpu`, the
+# bytes).  FP64 (`--precision fp64 --steps 1 --warmup 0 --no-cpu`, one dispatch): 2 x 4,815,118 KiB
+# fetched + 3,973,413 KiB written.  Mixed (`--precision mixed --steps 1 --warmup 1 --no-cpu`, the
 # hinted step: routed FP64 pass + mixed pass + redo pass): 2 x 6,702,294 KiB fetched + 4,368,279
 # KiB written; per 2,097,152 particles.
-TRAFFIC_BYTES_PER_PARTICLE_STEP = {(3, "fp64"): (2 * 4816650.7 + 3970019.4) * 1024 / 2097152,
+TRAFFIC_BYTES_PER_PARTICLE_STEP = {(3, "fp64"): (2 * 4815117.8 + 3973413.2) * 1024 / 2097152,
                                    (3, "mixed"): (2 * 6702293.56 + 4368279.28) * 1024 / 2097152}
 
 # The same for the EOS refresh kernel over the C3 particle set (`--workload helm-eos --steps 2
